@@ -1,0 +1,361 @@
+"""ctypes front end of the CPU f64 oracle (oracle/wcsph_oracle.cpp).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and by
+the CPU-baseline leg of bench.py, always as the checker. The product package
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+FLUID, BOUNDARY = 0, 1
+
+
+class OraParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("rest_density", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("gamma", C.c_double),
+        ("sound_speed", C.c_double),
+        ("visc_alpha", C.c_double),
+        ("visc_beta", C.c_double),
+        ("visc_eps", C.c_double),
+        ("xsph_eps", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("adapt_sound_speed", C.c_int32),
+    ]
+
+
+class OraDiag(C.Structure):
+    _fields_ = [
+        ("total_mass", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("kinetic_energy", C.c_double),
+        ("potential_energy", C.c_double),
+        ("mean_density_error", C.c_double),
+        ("max_density_error", C.c_double),
+        ("v_max", C.c_double),
+        ("a_max", C.c_double),
+        ("hash", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    d, i32, u64, vp = C.c_double, C.c_int32, C.c_uint64, C.c_void_p
+    pd = C.POINTER(C.c_double)
+    sig = {
+        "ora_w": (d, [d, d, i32]),
+        "ora_dwdr": (d, [d, d, i32]),
+        "ora_grad_w": (None, [pd, d, i32, pd]),
+        "ora_tait_pressure": (d, [d, d, d, d]),
+        "ora_ensure_sound_speed": (d, [d, d]),
+        "ora_pi_ij": (d, [d, d, d, d, d, pd, pd, pd, pd, d, d]),
+        "ora_floor_i32": (i32, [d]),
+        "ora_mass_from_spacing": (d, [d, d, i32]),
+        "ora_mass_calibrated_to_rest": (d, [d, d, d, i32]),
+        "ora_rng_new": (vp, [u64]),
+        "ora_rng_free": (None, [vp]),
+        "ora_rng_set_state": (None, [vp, C.POINTER(C.c_uint64)]),
+        "ora_splitmix64": (u64, [C.POINTER(C.c_uint64)]),
+        "ora_rng_next_u64": (u64, [vp]),
+        "ora_rng_uniform": (d, [vp, d, d]),
+        "ora_parts_new": (vp, [i32, d, d, d]),
+        "ora_parts_free": (None, [vp]),
+        "ora_parts_len": (u64, [vp]),
+        "ora_parts_push_fluid": (None, [vp, pd, pd]),
+        "ora_parts_push_boundary": (None, [vp, pd, d, pd]),
+        "ora_fill_box": (None, [vp, pd, u64, u64, u64, d, pd]),
+        "ora_fill_box_boundary": (None, [vp, pd, pd, d, u64]),
+        "ora_recomputed_akinci_psi": (None, [vp, i32]),
+        "ora_parts_jitter": (None, [vp, u64, d]),
+        "ora_parts_get": (None, [vp, vp, vp, vp, vp, vp]),
+        "ora_world_new": (vp, [C.POINTER(OraParams), u64, vp, vp, vp, vp, vp, i32, i32]),
+        "ora_world_free": (None, [vp]),
+        "ora_world_set_threads": (None, [vp, i32]),
+        "ora_world_step": (None, [vp, d, u64]),
+        "ora_world_rebuild_grid": (None, [vp]),
+        "ora_world_compute_density": (None, [vp]),
+        "ora_world_compute_pressure": (None, [vp]),
+        "ora_world_compute_forces": (None, [vp]),
+        "ora_world_apply_xsph": (None, [vp]),
+        "ora_world_time": (d, [vp]),
+        "ora_world_sound_speed": (d, [vp]),
+        "ora_world_set_sound_speed": (None, [vp, d]),
+        "ora_world_len": (u64, [vp]),
+        "ora_world_get_state": (None, [vp, vp, vp, vp, vp, vp, vp]),
+        "ora_world_set_state": (None, [vp, vp, vp]),
+        "ora_world_set_rho_p": (None, [vp, vp, vp]),
+        "ora_world_grid_cells": (u64, [vp]),
+        "ora_world_get_grid": (None, [vp, vp, vp, vp, vp]),
+        "ora_world_neighbours": (u64, [vp, u64, vp, u64]),
+        "ora_world_diagnostics": (None, [vp, d, C.POINTER(OraDiag)]),
+        "ora_fnv1a_f64": (u64, [vp, u64]),
+        "ora_fnv1a_bytes": (u64, [vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _pd(vals):
+    arr = (C.c_double * len(vals))(*[float(v) for v in vals])
+    return arr
+
+
+# --- closed forms -------------------------------------------------------------
+def w(r, h, dim=3):
+    return lib().ora_w(r, h, dim)
+
+
+def dwdr(r, h, dim=3):
+    return lib().ora_dwdr(r, h, dim)
+
+
+def grad_w(rij, h, dim=3):
+    out = (C.c_double * 3)()
+    lib().ora_grad_w(_pd(rij), h, dim, out)
+    return np.array(out[:])
+
+
+def tait_pressure(rho, rho0, c, gamma=7.0):
+    return lib().ora_tait_pressure(rho, rho0, c, gamma)
+
+
+def ensure_sound_speed(c, vmax):
+    return lib().ora_ensure_sound_speed(c, vmax)
+
+
+def pi_ij(alpha, beta, eps, h, c, xi, xj, vi, vj, rho_i, rho_j):
+    return lib().ora_pi_ij(alpha, beta, eps, h, c, _pd(xi), _pd(xj), _pd(vi), _pd(vj), rho_i, rho_j)
+
+
+def floor_i32(x):
+    return lib().ora_floor_i32(x)
+
+
+def mass_from_spacing(rho0, dx, dim=3):
+    return lib().ora_mass_from_spacing(rho0, dx, dim)
+
+
+def mass_calibrated_to_rest(rho0, dx, h, dim=3):
+    return lib().ora_mass_calibrated_to_rest(rho0, dx, h, dim)
+
+
+def fnv1a_f64(vals):
+    a = np.ascontiguousarray(vals, dtype=np.float64)
+    return lib().ora_fnv1a_f64(_p(a), a.size)
+
+
+def fnv1a_bytes(b: bytes):
+    a = np.frombuffer(b, dtype=np.uint8).copy() if len(b) else np.zeros(1, np.uint8)
+    return lib().ora_fnv1a_bytes(_p(a), len(b))
+
+
+class Rng:
+    def __init__(self, seed: int):
+        self.h = lib().ora_rng_new(seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_rng_free(self.h)
+
+    def set_state(self, s):
+        arr = (C.c_uint64 * 4)(*s)
+        lib().ora_rng_set_state(self.h, arr)
+
+    def next_u64(self):
+        return lib().ora_rng_next_u64(self.h)
+
+    def uniform(self, lo, hi):
+        return lib().ora_rng_uniform(self.h, lo, hi)
+
+
+def splitmix64_stream(seed: int, n: int):
+    st = C.c_uint64(seed)
+    return [lib().ora_splitmix64(C.byref(st)) for _ in range(n)]
+
+
+# --- particles / scenes ---------------------------------------------------------
+class Particles:
+    """Scene builder mirroring particles.rs (fill_box, fill_box_boundary, psi)."""
+
+    def __init__(self, dim, rest_density, mass, h):
+        self.dim, self.rest_density, self.mass, self.h = dim, rest_density, mass, h
+        self.hnd = lib().ora_parts_new(dim, rest_density, mass, h)
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_parts_free(self.hnd)
+
+    def __len__(self):
+        return lib().ora_parts_len(self.hnd)
+
+    def push_fluid(self, x, v=(0, 0, 0)):
+        lib().ora_parts_push_fluid(self.hnd, _pd(x), _pd(v))
+
+    def push_boundary(self, x, psi, n_out=(0, 0, 0)):
+        lib().ora_parts_push_boundary(self.hnd, _pd(x), psi, _pd(n_out))
+
+    def fill_box(self, origin, nx, ny, nz, dx, v0=(0, 0, 0)):
+        lib().ora_fill_box(self.hnd, _pd(origin), nx, ny, nz, dx, _pd(v0))
+
+    def fill_box_boundary(self, mn, mx, dx, layers=1):
+        lib().ora_fill_box_boundary(self.hnd, _pd(mn), _pd(mx), dx, layers)
+
+    def recompute_psi(self, strict=False):
+        lib().ora_recomputed_akinci_psi(self.hnd, 1 if strict else 0)
+
+    def jitter(self, seed, amp):
+        lib().ora_parts_jitter(self.hnd, seed, amp)
+
+    def arrays(self):
+        n = len(self)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        kind = np.zeros(n, np.uint8)
+        psi = np.zeros(n)
+        n_out = np.zeros((n, 3))
+        lib().ora_parts_get(self.hnd, _p(x), _p(v), _p(kind), _p(psi), _p(n_out))
+        return dict(x=x, v=v, kind=kind, psi=psi, n_out=n_out)
+
+
+STRICT, FAST = 0, 1
+
+
+def make_params(dim=3, rest_density=1000.0, mass=1.0, h=0.1, gamma=7.0, sound_speed=20.0,
+                visc_alpha=0.01, visc_beta=0.0, visc_eps=0.01, xsph_eps=0.25,
+                gravity=(0.0, 0.0, -9.80665), adapt_sound_speed=True):
+    p = OraParams()
+    p.dim, p.rest_density, p.mass, p.h = dim, rest_density, mass, h
+    p.gamma, p.sound_speed = gamma, sound_speed
+    p.visc_alpha, p.visc_beta, p.visc_eps = visc_alpha, visc_beta, visc_eps
+    p.xsph_eps = xsph_eps
+    for k in range(3):
+        p.gravity[k] = gravity[k]
+    p.adapt_sound_speed = 1 if adapt_sound_speed else 0
+    return p
+
+
+class World:
+    """Oracle WcsphWorld (wcsph.rs:56-370) over flat f64 arrays."""
+
+    def __init__(self, params: dict, x, kind, v=None, psi=None, n_out=None, mode=FAST, threads=1):
+        self.params = dict(params)
+        self.n = len(kind)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        kind = np.ascontiguousarray(kind, dtype=np.uint8)
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        n_out = None if n_out is None else np.ascontiguousarray(n_out, dtype=np.float64).reshape(-1)
+        self._prm = make_params(**params)
+        self.hnd = lib().ora_world_new(C.byref(self._prm), self.n, _p(x), _p(v), _p(kind), _p(psi), _p(n_out),
+                                       mode, threads)
+        self.kind = kind.copy()
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_world_free(self.hnd)
+
+    def set_threads(self, t):
+        lib().ora_world_set_threads(self.hnd, t)
+
+    def step(self, dt, nsteps=1):
+        lib().ora_world_step(self.hnd, dt, nsteps)
+
+    def rebuild_grid(self):
+        lib().ora_world_rebuild_grid(self.hnd)
+
+    def compute_density(self):
+        lib().ora_world_compute_density(self.hnd)
+
+    def compute_pressure(self):
+        lib().ora_world_compute_pressure(self.hnd)
+
+    def compute_forces(self):
+        lib().ora_world_compute_forces(self.hnd)
+
+    def apply_xsph(self):
+        lib().ora_world_apply_xsph(self.hnd)
+
+    @property
+    def t(self):
+        return lib().ora_world_time(self.hnd)
+
+    @property
+    def sound_speed(self):
+        return lib().ora_world_sound_speed(self.hnd)
+
+    @sound_speed.setter
+    def sound_speed(self, c):
+        lib().ora_world_set_sound_speed(self.hnd, c)
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "a", "boundary_force")}
+        out["rho"] = np.zeros(n)
+        out["p"] = np.zeros(n)
+        lib().ora_world_get_state(self.hnd, _p(out["x"]), _p(out["v"]), _p(out["a"]), _p(out["rho"]),
+                                  _p(out["p"]), _p(out["boundary_force"]))
+        return out
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        lib().ora_world_set_state(self.hnd, _p(xa), _p(va))
+
+    def set_rho_p(self, rho=None, p=None):
+        ra = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        pa = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        lib().ora_world_set_rho_p(self.hnd, _p(ra), _p(pa))
+
+    def grid(self):
+        n = self.n
+        m = lib().ora_world_grid_cells(self.hnd)
+        keys = np.zeros((n, 3), np.int32)
+        order = np.zeros(n, np.uint64)
+        occ = np.zeros((m, 3), np.int32)
+        rng = np.zeros((m, 2), np.uint64)
+        lib().ora_world_get_grid(self.hnd, _p(keys), _p(order), _p(occ), _p(rng))
+        return dict(keys=keys, order=order, occupied=occ, ranges=rng)
+
+    def neighbours(self, i):
+        n = lib().ora_world_neighbours(self.hnd, i, None, 0)
+        out = np.zeros(max(n, 1), np.uint64)
+        lib().ora_world_neighbours(self.hnd, i, _p(out), n)
+        return out[:n]
+
+    def diagnostics(self, g=9.80665):
+        d = OraDiag()
+        lib().ora_world_diagnostics(self.hnd, g, C.byref(d))
+        return dict(total_mass=d.total_mass, momentum=np.array(d.momentum[:]), kinetic_energy=d.kinetic_energy,
+                    potential_energy=d.potential_energy, mean_density_error=d.mean_density_error,
+                    max_density_error=d.max_density_error, v_max=d.v_max, a_max=d.a_max, hash=d.hash)

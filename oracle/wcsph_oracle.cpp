@@ -1,0 +1,1110 @@
+// CPU f64 restatement of pe-fluid's WCSPH step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the correctness oracle and the timed
+// CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker. The product path
+// (physics-engine_amd/) never links, loads or calls anything here.
+//
+// Every function follows the reference operation for operation: the same loop
+// orders, the same summation orders and the same association of every
+// floating-point expression. It must be compiled with -ffp-contract=off and
+// without -ffast-math, because the reference forbids FMA and re-association.
+// Paths below are relative to the reference checkout.
+//
+// Parity pins (how each part is checked, see tests/test_oracle_*.py):
+//   cubic spline w / grad_w ........ closed form in exact rational arithmetic
+//                                    (Python fractions) + the reference's own
+//                                    KATs (partition of unity, gradient sums).
+//   floor_i32 / key_of / rebuild ... brute-force sort of (key, index) in numpy,
+//                                    and grid.rs:159-181 determinism test.
+//   neighbours ..................... brute-force O(N^2) neighbour sets.
+//   Tait EOS ....................... closed form, p(rho0) = 0, hydrostatic KAT.
+//                                    Last-bit agreement with the reference's
+//                                    `libm` 0.2.16 pow is parity unpinned.
+//   pi_ij .......................... closed form; KE non-increase KAT.
+//   compute_forces ................. pair-momentum KAT, antisymmetry, strict vs
+//                                    gather-form bit identity.
+//   step ........................... mass conservation KAT, hydrostatic order KAT.
+//                                    The bitwise f64 trajectory is parity unpinned
+//                                    (no reference hash or golden vector exists).
+//   FNV-1a hash .................... published FNV-1a 64 test vectors.
+//   xoshiro256** / SplitMix64 ...... published reference outputs.
+//   fill_box / fill_box_boundary ... closed-form particle counts.
+//   recomputed_akinci_psi .......... O(B^2) strict vs grid form bit identity.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace ora {
+
+// ---------------------------------------------------------------------------
+// pe-math Vec3 (crates/pe-math/src/vec3.rs:8-16, 63-83, ops 88+).
+// norm_squared = (x*x + y*y) + z*z, left to right; norm = sqrt(norm_squared).
+// ---------------------------------------------------------------------------
+struct V3 {
+  double x, y, z;
+};
+static inline V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+static inline V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 mul(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+static inline double norm2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double norm(V3 a) { return std::sqrt(norm2(a)); }
+
+enum { KIND_FLUID = 0, KIND_BOUNDARY = 1 };
+enum { DIM2 = 2, DIM3 = 3 };
+
+static const double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Cubic spline (crates/pe-fluid/src/kernels.rs:32-116).
+// ---------------------------------------------------------------------------
+static inline double cubic_alpha(double h, int dim) {  // kernels.rs:79-84
+  if (dim == DIM2) return 10.0 / (7.0 * PI * h * h);
+  return 1.0 / (PI * h * h * h);
+}
+
+static inline double w(double r, double h, int dim) {  // kernels.rs:40-50, 86-100
+  if (h <= 0.0 || r < 0.0) return 0.0;
+  double q = r / h;
+  if (q >= 2.0) return 0.0;
+  double a = cubic_alpha(h, dim);
+  if (q < 1.0) return a * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+  double t = 2.0 - q;
+  return a * 0.25 * t * t * t;
+}
+
+static inline double dwdr(double r, double h, int dim) {  // kernels.rs:102-116
+  double q = r / h;
+  if (q >= 2.0 || q <= 0.0) return 0.0;
+  double a = cubic_alpha(h, dim);
+  double dwdq;
+  if (q < 1.0) {
+    dwdq = a * (-3.0 * q + 2.25 * q * q);
+  } else {
+    double t = 2.0 - q;
+    dwdq = a * (-0.75 * t * t);
+  }
+  return dwdq / h;
+}
+
+static inline V3 grad_w(V3 rij, double h, int dim) {  // kernels.rs:56-75
+  if (h <= 0.0) return v3(0, 0, 0);
+  double r2 = norm2(rij);
+  double r = std::sqrt(r2);
+  double d = dwdr(r, h, dim);
+  if (r < 1e-30) return v3(0, 0, 0);
+  return mul(rij, -d / r);
+}
+
+// ---------------------------------------------------------------------------
+// Tait EOS (crates/pe-fluid/src/eos.rs:59-81).
+// ---------------------------------------------------------------------------
+static inline double stiffness_b(double rho0, double c, double gamma) {  // eos.rs:59-61
+  return rho0 * c * c / gamma;
+}
+static inline double tait_pressure(double rho, double rho0, double c, double gamma) {  // eos.rs:65-69
+  double ratio = rho / rho0;
+  double b = stiffness_b(rho0, c, gamma);
+  return b * (std::pow(ratio, gamma) - 1.0);
+}
+static inline double ensure_sound_speed(double c, double vmax) {  // eos.rs:74-81
+  double min_c = 10.0 * std::fmax(vmax, 0.0);
+  return (c < min_c) ? min_c : c;
+}
+
+// ---------------------------------------------------------------------------
+// Monaghan viscosity (crates/pe-fluid/src/viscosity.rs:51-73).
+// ---------------------------------------------------------------------------
+struct Visc {
+  double alpha, beta, eps;
+};
+static inline double pi_ij(Visc v, double h, double c, V3 xi, V3 xj, V3 vi, V3 vj, double rho_i,
+                           double rho_j) {
+  V3 rij = sub(xi, xj);
+  V3 vij = sub(vi, vj);
+  double vr = dot(vij, rij);
+  if (vr >= 0.0) return 0.0;
+  double r2 = norm2(rij);
+  double mu = h * vr / (r2 + v.eps * h * h);
+  double rho_bar = 0.5 * (rho_i + rho_j);
+  double c_bar = c;
+  return (-v.alpha * c_bar * mu + v.beta * mu * mu) / rho_bar;
+}
+
+// ---------------------------------------------------------------------------
+// Particles (crates/pe-fluid/src/particles.rs:20-134).
+// ---------------------------------------------------------------------------
+struct Particles {
+  int dim = DIM3;
+  double rest_density = 1000.0, mass = 1.0, h = 0.1;
+  std::vector<V3> x, v, a;
+  std::vector<double> rho, p;
+  std::vector<uint8_t> kind;
+  std::vector<double> psi;
+  std::vector<V3> n_out;
+
+  size_t len() const { return x.size(); }
+  void push_fluid(V3 px, V3 pv) {  // particles.rs:90-99
+    x.push_back(px);
+    v.push_back(pv);
+    a.push_back(v3(0, 0, 0));
+    rho.push_back(rest_density);
+    p.push_back(0.0);
+    kind.push_back(KIND_FLUID);
+    psi.push_back(0.0);
+    n_out.push_back(v3(0, 0, 0));
+  }
+  void push_boundary_oriented(V3 px, double ps, V3 n) {  // particles.rs:107-116
+    x.push_back(px);
+    v.push_back(v3(0, 0, 0));
+    a.push_back(v3(0, 0, 0));
+    rho.push_back(rest_density);
+    p.push_back(0.0);
+    kind.push_back(KIND_BOUNDARY);
+    psi.push_back(ps);
+    n_out.push_back(n);
+  }
+  bool boundary_sees_fluid(size_t b, V3 xf) const {  // particles.rs:126-134
+    V3 n = n_out[b];
+    double n2 = norm2(n);
+    if (n2 <= 0.0) return true;
+    double eps = 0.15 * h * std::sqrt(std::fmax(n2, 0.0));
+    return dot(sub(xf, x[b]), n) > -eps;
+  }
+  size_t fluid_count() const {
+    size_t c = 0;
+    for (auto k : kind) c += (k == KIND_FLUID);
+    return c;
+  }
+};
+
+static double mass_from_spacing(double rho0, double dx, int dim) {  // particles.rs:166-171
+  if (dim == DIM2) return rho0 * dx * dx;
+  return rho0 * dx * dx * dx;
+}
+
+static double mass_calibrated_to_rest(double rho0, double dx, double h, int dim) {  // particles.rs:178-215
+  double support = 2.0 * h;
+  int n = (int)std::ceil(support / dx) + 1;
+  double sum_w = w(0.0, h, dim);
+  for (int ix = -n; ix <= n; ++ix)
+    for (int iy = -n; iy <= n; ++iy) {
+      int zlo = (dim == DIM2) ? 0 : -n, zhi = (dim == DIM2) ? 0 : n;
+      for (int iz = zlo; iz <= zhi; ++iz) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        double r = std::sqrt(((double)ix * dx) * ((double)ix * dx) + ((double)iy * dx) * ((double)iy * dx) +
+                             ((double)iz * dx) * ((double)iz * dx));
+        if (r < support) sum_w += w(r, h, dim);
+      }
+    }
+  if (sum_w < 1e-30) return mass_from_spacing(rho0, dx, dim);
+  return rho0 / sum_w;
+}
+
+static void fill_box(Particles& P, V3 origin, size_t nx, size_t ny, size_t nz, double dx, V3 v0) {
+  // particles.rs:220-245; loop order iz -> iy -> ix.
+  if (P.dim == DIM2) nz = 1;
+  for (size_t iz = 0; iz < nz; ++iz)
+    for (size_t iy = 0; iy < ny; ++iy)
+      for (size_t ix = 0; ix < nx; ++ix)
+        P.push_fluid(v3(origin.x + (double)ix * dx, origin.y + (double)iy * dx, origin.z + (double)iz * dx), v0);
+}
+
+static size_t round_count(double span, double dx) {
+  // Rust `(f64).round() as usize`: round half away from zero, negative -> 0.
+  double r = std::round(span / dx);
+  if (!(r > 0.0)) return 0;
+  return (size_t)r;
+}
+
+static void fill_box_boundary(Particles& P, V3 mn, V3 mx, double dx, size_t layers) {
+  // particles.rs:286-347.
+  double psi = (P.dim == DIM2) ? dx * dx : dx * dx * dx;
+  if (layers < 1) layers = 1;
+  V3 zero = v3(0, 0, 0);
+  if (P.dim == DIM2) {
+    size_t nx = round_count(mx.x - mn.x, dx) + 1;
+    size_t ny = round_count(mx.y - mn.y, dx) + 1;
+    for (size_t layer = 0; layer < layers; ++layer) {
+      double off = (double)layer * dx;
+      for (size_t i = 0; i < nx; ++i) {
+        double x = mn.x + (double)i * dx;
+        P.push_boundary_oriented(v3(x, mn.y - off, 0.0), psi, zero);
+        P.push_boundary_oriented(v3(x, mx.y + off, 0.0), psi, zero);
+      }
+      for (size_t j = 1; j + 1 < ny; ++j) {
+        double y = mn.y + (double)j * dx;
+        P.push_boundary_oriented(v3(mn.x - off, y, 0.0), psi, zero);
+        P.push_boundary_oriented(v3(mx.x + off, y, 0.0), psi, zero);
+      }
+    }
+    return;
+  }
+  size_t nx = round_count(mx.x - mn.x, dx) + 1;
+  size_t ny = round_count(mx.y - mn.y, dx) + 1;
+  size_t nz = round_count(mx.z - mn.z, dx) + 1;
+  for (size_t layer = 0; layer < layers; ++layer) {
+    double off = (double)layer * dx;
+    for (size_t i = 0; i < nx; ++i)
+      for (size_t j = 0; j < ny; ++j) {
+        double x = mn.x + (double)i * dx, y = mn.y + (double)j * dx;
+        P.push_boundary_oriented(v3(x, y, mn.z - off), psi, zero);
+        P.push_boundary_oriented(v3(x, y, mx.z + off), psi, zero);
+      }
+    for (size_t i = 0; i < nx; ++i)
+      for (size_t k = 1; k + 1 < nz; ++k) {
+        double x = mn.x + (double)i * dx, z = mn.z + (double)k * dx;
+        P.push_boundary_oriented(v3(x, mn.y - off, z), psi, zero);
+        P.push_boundary_oriented(v3(x, mx.y + off, z), psi, zero);
+      }
+    for (size_t j = 1; j + 1 < ny; ++j)
+      for (size_t k = 1; k + 1 < nz; ++k) {
+        double y = mn.y + (double)j * dx, z = mn.z + (double)k * dx;
+        P.push_boundary_oriented(v3(mn.x - off, y, z), psi, zero);
+        P.push_boundary_oriented(v3(mx.x + off, y, z), psi, zero);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform grid (crates/pe-fluid/src/grid.rs:12-157).
+// ---------------------------------------------------------------------------
+struct Key {
+  int32_t ix, iy, iz;
+};
+static inline bool key_lt(const Key& a, const Key& b) {  // derived Ord: ix, then iy, then iz
+  if (a.ix != b.ix) return a.ix < b.ix;
+  if (a.iy != b.iy) return a.iy < b.iy;
+  return a.iz < b.iz;
+}
+static inline bool key_eq(const Key& a, const Key& b) { return a.ix == b.ix && a.iy == b.iy && a.iz == b.iz; }
+
+static inline int32_t floor_i32(double x) {  // grid.rs:144-157
+  if (!std::isfinite(x)) return 0;
+  const double lo = (double)std::numeric_limits<int32_t>::min() + 1.0;
+  const double hi = (double)std::numeric_limits<int32_t>::max() - 1.0;
+  double c = x < lo ? lo : (x > hi ? hi : x);
+  int32_t t = (int32_t)c;  // truncation toward zero; in range by the clamp
+  if ((double)t > c) return t - 1;  // t > INT32_MIN here, so no saturation is needed
+  return t;
+}
+
+struct Grid {
+  double cell_size = 1.0;
+  int dim = DIM3;
+  std::vector<uint64_t> order;
+  std::vector<Key> keys;
+  std::vector<Key> occupied;
+  std::vector<std::pair<uint64_t, uint64_t>> ranges;
+
+  Key key_of(V3 p) const {  // grid.rs:62-71
+    double inv = 1.0 / cell_size;
+    Key k;
+    k.ix = floor_i32(p.x * inv);
+    k.iy = floor_i32(p.y * inv);
+    k.iz = (dim == DIM2) ? 0 : floor_i32(p.z * inv);
+    return k;
+  }
+  void rebuild(const std::vector<V3>& pos) {  // grid.rs:76-107
+    size_t n = pos.size();
+    keys.resize(n);
+    order.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      keys[i] = key_of(pos[i]);
+      order[i] = i;
+    }
+    const std::vector<Key>& K = keys;
+    std::sort(order.begin(), order.end(), [&K](uint64_t a, uint64_t b) {
+      if (key_lt(K[a], K[b])) return true;
+      if (key_lt(K[b], K[a])) return false;
+      return a < b;
+    });
+    occupied.clear();
+    ranges.clear();
+    if (n == 0) return;
+    size_t start = 0;
+    Key cur = keys[order[0]];
+    for (size_t s = 1; s < n; ++s) {
+      Key k = keys[order[s]];
+      if (!key_eq(k, cur)) {
+        occupied.push_back(cur);
+        ranges.push_back({start, s});
+        start = s;
+        cur = k;
+      }
+    }
+    occupied.push_back(cur);
+    ranges.push_back({start, n});
+  }
+  bool cell_range(Key k, uint64_t* a, uint64_t* b) const {  // grid.rs:111-114 (binary search)
+    size_t lo = 0, hi = occupied.size();
+    while (lo < hi) {
+      size_t mid = lo + (hi - lo) / 2;
+      if (key_lt(occupied[mid], k))
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo < occupied.size() && key_eq(occupied[lo], k)) {
+      *a = ranges[lo].first;
+      *b = ranges[lo].second;
+      return true;
+    }
+    return false;
+  }
+  // grid.rs:122-140: 27 (or 9) cells, dz outer -> dy -> dx inner; self included.
+  // Calls f(j) for each candidate in the reference order.
+  template <class F>
+  void for_each_neighbour(size_t i, F&& f) const {
+    Key key = keys[i];
+    int zlo = (dim == DIM2) ? 0 : -1, zhi = (dim == DIM2) ? 0 : 1;
+    for (int dz = zlo; dz <= zhi; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          // Rust i32 `+` on keys: the reference panics on overflow in debug and
+          // wraps in release; keys are clamped to +-(2^31-2) so +-1 never overflows.
+          Key nk{key.ix + dx, key.iy + dy, key.iz + dz};
+          uint64_t a, b;
+          if (cell_range(nk, &a, &b))
+            for (uint64_t s = a; s < b; ++s) f((size_t)order[s]);
+        }
+  }
+  void neighbours(size_t i, std::vector<uint64_t>& out) const {
+    for_each_neighbour(i, [&out](size_t j) { out.push_back(j); });
+  }
+};
+
+// ---------------------------------------------------------------------------
+// WCSPH world (crates/pe-fluid/src/wcsph.rs:22-370).
+// ---------------------------------------------------------------------------
+struct Params {
+  double gamma = 7.0, sound_speed = 20.0;  // eos (rest density lives in Particles)
+  Visc visc{0.01, 0.0, 0.01};
+  double xsph_eps = 0.25;
+  V3 gravity{0.0, 0.0, -9.80665};
+  bool adapt_sound_speed = true;
+};
+
+// Evaluation modes. STRICT walks every candidate exactly like the reference,
+// including the materialised neigh_lists of rebuild_grid. FAST skips the
+// candidates whose kernel value and gradient are exactly zero (q >= 2) and does
+// not materialise neigh_lists; skipped terms add +-0.0 to accumulators that can
+// never hold -0.0, so FAST is bit-identical to STRICT (checked by a test).
+enum { MODE_STRICT = 0, MODE_FAST = 1 };
+
+struct World {
+  Particles P;
+  Grid grid;
+  Params params;
+  double t = 0.0;
+  int mode = MODE_FAST;
+  int threads = 1;  // >1 only for FAST: per-particle gathers are order-preserving
+  std::vector<std::vector<uint64_t>> neigh_lists;
+  std::vector<uint64_t> neigh;
+  std::vector<V3> xsph_dv;
+  std::vector<V3> boundary_force;
+
+  double cut2() const {
+    // r2 above this bound guarantees q = sqrt(r2)/h >= 2 after rounding.
+    double s = 2.0 * P.h;
+    return s * s * (1.0 + 1e-9);
+  }
+
+  void init() {  // wcsph.rs:82-99
+    grid.cell_size = 2.0 * P.h;
+    grid.dim = P.dim;
+    neigh_lists.assign(P.len(), {});
+    boundary_force.assign(P.len(), v3(0, 0, 0));
+  }
+
+  void rebuild_grid() {  // wcsph.rs:102-114
+    grid.rebuild(P.x);
+    size_t n = P.len();
+    if (mode == MODE_STRICT) {
+      if (neigh_lists.size() != n) neigh_lists.resize(n);
+      for (size_t i = 0; i < n; ++i) {
+        neigh_lists[i].clear();
+        if (P.kind[i] == KIND_FLUID) grid.neighbours(i, neigh_lists[i]);
+      }
+    }
+  }
+
+  double density_of(size_t i, std::vector<uint64_t>* scratch) const {  // wcsph.rs:130-152
+    const double h = P.h, mass = P.mass, rho0 = P.rest_density;
+    const int dim = P.dim;
+    V3 xi = P.x[i];
+    double rho = 0.0;
+    rho += mass * w(0.0, h, dim);
+    auto body = [&](size_t j) {
+      if (j == i) return;
+      V3 d = sub(P.x[j], xi);
+      if (mode == MODE_FAST && norm2(d) > cut2()) return;
+      double r = norm(d);
+      double wij = w(r, h, dim);
+      if (P.kind[j] == KIND_FLUID) {
+        rho += mass * wij;
+      } else if (P.boundary_sees_fluid(j, xi)) {
+        rho += rho0 * P.psi[j] * wij;
+      }
+    };
+    if (mode == MODE_STRICT) {
+      scratch->clear();
+      grid.neighbours(i, *scratch);
+      for (uint64_t j : *scratch) body((size_t)j);
+    } else {
+      grid.for_each_neighbour(i, body);
+    }
+    return std::fmax(rho, 1e-12);
+  }
+
+  void compute_density() {  // wcsph.rs:117-154
+    size_t n = P.len();
+    if (mode == MODE_STRICT) {
+      for (size_t i = 0; i < n; ++i) {
+        if (P.kind[i] != KIND_FLUID) {
+          P.rho[i] = P.rest_density;
+          continue;
+        }
+        P.rho[i] = density_of(i, &neigh);
+      }
+      return;
+    }
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) if (threads > 1)
+    for (long long ii = 0; ii < (long long)n; ++ii) {
+      size_t i = (size_t)ii;
+      P.rho[i] = (P.kind[i] != KIND_FLUID) ? P.rest_density : density_of(i, nullptr);
+    }
+  }
+
+  double max_speed() const {  // viscosity.rs:89-100
+    double m = 0.0;
+    for (size_t i = 0; i < P.len(); ++i)
+      if (P.kind[i] == KIND_FLUID) {
+        double s = norm2(P.v[i]);
+        if (s > m) m = s;
+      }
+    return std::sqrt(m);
+  }
+
+  void compute_pressure() {  // wcsph.rs:157-178
+    double c = params.sound_speed;
+    if (params.adapt_sound_speed) {
+      c = ensure_sound_speed(c, max_speed());
+      params.sound_speed = c;
+    }
+    for (size_t i = 0; i < P.len(); ++i)
+      P.p[i] = (P.kind[i] == KIND_FLUID) ? tait_pressure(P.rho[i], P.rest_density, params.sound_speed, params.gamma)
+                                         : 0.0;
+  }
+
+  // The pair force f_ij that particle i (fluid) adds for fluid j (wcsph.rs:218-240).
+  V3 fluid_pair_force(size_t i, size_t j, V3 gw) const {
+    const double mass = P.mass;
+    double rho_i = P.rho[i], p_i = P.p[i], rho_j = P.rho[j], p_j = P.p[j];
+    double press = p_i / (rho_i * rho_i) + p_j / (rho_j * rho_j);
+    V3 f_press = mul(gw, -mass * press);
+    double pi = pi_ij(params.visc, P.h, params.sound_speed, P.x[i], P.x[j], P.v[i], P.v[j], rho_i, rho_j);
+    V3 f_visc = mul(gw, -mass * pi);
+    return add(f_press, f_visc);
+  }
+  // The Akinci force on fluid i from boundary j (wcsph.rs:250-251).
+  V3 boundary_pair_force(size_t i, size_t j, V3 gw) const {
+    double rho_i = P.rho[i], p_i = P.p[i];
+    double press = p_i / (rho_i * rho_i);
+    return mul(gw, -(P.rest_density * P.psi[j]) * press);
+  }
+
+  void compute_forces_scatter() {  // wcsph.rs:181-259, the reference's own scatter order
+    size_t n = P.len();
+    const double h = P.h;
+    const int dim = P.dim;
+    for (auto& a : P.a) a = v3(0, 0, 0);
+    if (boundary_force.size() != n) boundary_force.resize(n);
+    for (auto& f : boundary_force) f = v3(0, 0, 0);
+    for (size_t i = 0; i < n; ++i) {
+      if (P.kind[i] != KIND_FLUID) continue;
+      V3 xi = P.x[i];
+      auto body = [&](size_t j) {
+        if (j == i) return;
+        V3 rij = sub(P.x[j], xi);
+        if (mode == MODE_FAST && norm2(rij) > cut2()) return;
+        double r = norm(rij);
+        if (r < 1e-30) return;
+        V3 gw = grad_w(rij, h, dim);
+        if (P.kind[j] == KIND_FLUID) {
+          if (j < i) return;
+          V3 f = fluid_pair_force(i, j, gw);
+          P.a[i] = add(P.a[i], f);
+          P.a[j] = sub(P.a[j], f);
+        } else {
+          if (!P.boundary_sees_fluid(j, xi)) return;
+          V3 f = boundary_pair_force(i, j, gw);
+          P.a[i] = add(P.a[i], f);
+          boundary_force[j] = add(boundary_force[j], mul(f, -P.mass));
+        }
+      };
+      if (mode == MODE_STRICT) {
+        neigh.clear();
+        grid.neighbours(i, neigh);
+        for (uint64_t j : neigh) body((size_t)j);
+      } else {
+        grid.for_each_neighbour(i, body);
+      }
+      P.a[i] = add(P.a[i], params.gravity);
+    }
+  }
+
+  // Gather form of the same sums, bit-identical to the scatter form: particle
+  // i first receives -f from every fluid i' < i in ascending i' (the order in
+  // which the scatter loop visits them), then its own terms in candidate order,
+  // then gravity. Boundary reactions are folded in ascending fluid index.
+  void compute_forces_gather() {
+    size_t n = P.len();
+    const double h = P.h;
+    const int dim = P.dim;
+    if (boundary_force.size() != n) boundary_force.resize(n);
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
+      std::vector<uint64_t> lower;
+#pragma omp for schedule(dynamic, 128)
+      for (long long ii = 0; ii < (long long)n; ++ii) {
+        size_t i = (size_t)ii;
+        V3 xi = P.x[i];
+        lower.clear();
+        if (P.kind[i] == KIND_FLUID) {
+          // Fluid i' < i holding i as a candidate are exactly i's fluid candidates j < i.
+          grid.for_each_neighbour(i, [&](size_t j) {
+            if (j < i && P.kind[j] == KIND_FLUID) lower.push_back(j);
+          });
+          std::sort(lower.begin(), lower.end());
+          V3 a = v3(0, 0, 0);
+          for (uint64_t jj : lower) {
+            size_t j = (size_t)jj;  // the scatter loop of j subtracts f_{j i}
+            V3 rji = sub(xi, P.x[j]);
+            if (norm2(rji) > cut2()) continue;
+            double r = norm(rji);
+            if (r < 1e-30) continue;
+            V3 gw = grad_w(rji, h, dim);
+            a = sub(a, fluid_pair_force(j, i, gw));
+          }
+          grid.for_each_neighbour(i, [&](size_t j) {
+            if (j == i) return;
+            V3 rij = sub(P.x[j], xi);
+            if (norm2(rij) > cut2()) return;
+            double r = norm(rij);
+            if (r < 1e-30) return;
+            V3 gw = grad_w(rij, h, dim);
+            if (P.kind[j] == KIND_FLUID) {
+              if (j < i) return;
+              a = add(a, fluid_pair_force(i, j, gw));
+            } else {
+              if (!P.boundary_sees_fluid(j, xi)) return;
+              a = add(a, boundary_pair_force(i, j, gw));
+            }
+          });
+          P.a[i] = add(a, params.gravity);
+        } else {
+          // Boundary j: reactions from fluid i in ascending i.
+          grid.for_each_neighbour(i, [&](size_t j) {
+            if (P.kind[j] == KIND_FLUID) lower.push_back(j);
+          });
+          std::sort(lower.begin(), lower.end());
+          V3 f = v3(0, 0, 0);
+          for (uint64_t jj : lower) {
+            size_t fi = (size_t)jj;
+            V3 rij = sub(xi, P.x[fi]);  // from the fluid particle's view: x_b - x_f
+            if (norm2(rij) > cut2()) continue;
+            double r = norm(rij);
+            if (r < 1e-30) continue;
+            if (!P.boundary_sees_fluid(i, P.x[fi])) continue;
+            V3 gw = grad_w(rij, h, dim);
+            f = add(f, mul(boundary_pair_force(fi, i, gw), -P.mass));
+          }
+          boundary_force[i] = f;
+          P.a[i] = v3(0, 0, 0);
+        }
+      }
+    }
+  }
+
+  void compute_forces() {
+    if (mode == MODE_STRICT || threads <= 1)
+      compute_forces_scatter();
+    else
+      compute_forces_gather();
+  }
+
+  V3 xsph_of(size_t i, std::vector<uint64_t>* scratch) const {  // wcsph.rs:297-316
+    const double h = P.h, mass = P.mass;
+    const int dim = P.dim;
+    V3 xi = P.x[i], vi = P.v[i];
+    double rho_i = P.rho[i];
+    V3 dv = v3(0, 0, 0);
+    auto body = [&](size_t j) {
+      if (j == i || P.kind[j] != KIND_FLUID) return;
+      V3 d = sub(P.x[j], xi);
+      if (mode == MODE_FAST && norm2(d) > cut2()) return;
+      double r = norm(d);
+      double wij = w(r, h, dim);
+      double rho_bar = 0.5 * (rho_i + P.rho[j]);
+      dv = add(dv, mul(sub(P.v[j], vi), mass / rho_bar * wij));
+    };
+    if (mode == MODE_STRICT) {
+      scratch->clear();
+      grid.neighbours(i, *scratch);
+      for (uint64_t j : *scratch) body((size_t)j);
+    } else {
+      grid.for_each_neighbour(i, body);
+    }
+    return mul(dv, params.xsph_eps);
+  }
+
+  void apply_xsph() {  // wcsph.rs:283-323
+    double eps = params.xsph_eps;
+    if (eps == 0.0) return;
+    size_t n = P.len();
+    xsph_dv.assign(n, v3(0, 0, 0));
+    if (mode == MODE_STRICT) {
+      for (size_t i = 0; i < n; ++i)
+        if (P.kind[i] == KIND_FLUID) xsph_dv[i] = xsph_of(i, &neigh);
+    } else {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) if (threads > 1)
+      for (long long ii = 0; ii < (long long)n; ++ii) {
+        size_t i = (size_t)ii;
+        if (P.kind[i] == KIND_FLUID) xsph_dv[i] = xsph_of(i, nullptr);
+      }
+    }
+    for (size_t i = 0; i < n; ++i)
+      if (P.kind[i] == KIND_FLUID) P.v[i] = add(P.v[i], xsph_dv[i]);
+  }
+
+  void step(double dt) {  // wcsph.rs:326-347
+    rebuild_grid();
+    compute_density();
+    compute_pressure();
+    compute_forces();
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (P.kind[i] != KIND_FLUID) continue;
+      P.v[i] = add(P.v[i], mul(P.a[i], dt));
+      P.x[i] = add(P.x[i], mul(P.v[i], dt));
+      if (P.dim == DIM2) {
+        P.x[i].z = 0.0;
+        P.v[i].z = 0.0;
+      }
+    }
+    rebuild_grid();
+    compute_density();
+    apply_xsph();
+    t += dt;
+  }
+
+  double a_max() const {  // wcsph.rs:361-369
+    double m = 0.0;
+    for (size_t i = 0; i < P.len(); ++i)
+      if (P.kind[i] == KIND_FLUID) m = std::fmax(m, norm2(P.a[i]));
+    return std::sqrt(m);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Akinci boundary volumes (particles.rs:252-279). The STRICT form is the
+// reference's O(B^2) loop; the grid form visits the same contributing partners
+// (r < support) in the same ascending index order, so it is bit-identical.
+// ---------------------------------------------------------------------------
+static void recomputed_akinci_psi(Particles& P, bool strict) {
+  const double h = P.h, support = 2.0 * h;
+  const int dim = P.dim;
+  std::vector<size_t> bound;
+  for (size_t i = 0; i < P.len(); ++i)
+    if (P.kind[i] == KIND_BOUNDARY) bound.push_back(i);
+  if (strict) {
+    for (size_t bi : bound) {
+      V3 xb = P.x[bi];
+      double delta = w(0.0, h, dim);
+      for (size_t bj : bound) {
+        if (bj == bi) continue;
+        double r = norm(sub(P.x[bj], xb));
+        if (r < support) delta += w(r, h, dim);
+      }
+      P.psi[bi] = (delta > 1e-30) ? 1.0 / delta : 0.0;
+    }
+    return;
+  }
+  // Grid over boundary samples only, cell = support: every partner with
+  // r < support lies in the 27-cell stencil.
+  std::vector<V3> bx(bound.size());
+  for (size_t k = 0; k < bound.size(); ++k) bx[k] = P.x[bound[k]];
+  Grid g;
+  g.cell_size = support;
+  g.dim = dim;
+  g.rebuild(bx);
+  std::vector<double> out(bound.size());
+#pragma omp parallel
+  {
+    std::vector<uint64_t> cand;
+#pragma omp for schedule(dynamic, 256)
+    for (long long kk = 0; kk < (long long)bound.size(); ++kk) {
+      size_t k = (size_t)kk;
+      cand.clear();
+      g.neighbours(k, cand);
+      std::sort(cand.begin(), cand.end());  // ascending boundary rank == ascending particle index
+      V3 xb = bx[k];
+      double delta = w(0.0, h, dim);
+      for (uint64_t c : cand) {
+        if (c == k) continue;
+        double r = norm(sub(bx[c], xb));
+        if (r < support) delta += w(r, h, dim);
+      }
+      out[k] = (delta > 1e-30) ? 1.0 / delta : 0.0;
+    }
+  }
+  for (size_t k = 0; k < bound.size(); ++k) P.psi[bound[k]] = out[k];
+}
+
+// ---------------------------------------------------------------------------
+// xoshiro256** with SplitMix64 seeding (crates/pe-math/src/rng.rs:14-72).
+// ---------------------------------------------------------------------------
+struct Rng {
+  uint64_t s[4];
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  static uint64_t splitmix64(uint64_t& st) {
+    st += 0x9e3779b97f4a7c15ULL;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+  }
+  explicit Rng(uint64_t seed) {
+    uint64_t sm = seed;
+    for (auto& v : s) v = splitmix64(sm);
+    if (!s[0] && !s[1] && !s[2] && !s[3]) s[0] = 0x9e3779b97f4a7c15ULL;
+  }
+  uint64_t next_u64() {
+    uint64_t result = rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+  }
+  double next_f64() { return (double)(next_u64() >> 11) * (1.0 / (double)(1ULL << 53)); }
+  double uniform(double lo, double hi) { return lo + (hi - lo) * next_f64(); }
+};
+
+// FNV-1a over little-endian f64 bits of fluid (x, v) (diagnostics.rs:7-33).
+static uint64_t particles_hash64(const Particles& P) {
+  uint64_t hsh = 0xcbf29ce484222325ULL;
+  auto eat = [&hsh](double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    for (int k = 0; k < 8; ++k) {
+      hsh ^= (b >> (8 * k)) & 0xffULL;
+      hsh *= 0x100000001b3ULL;
+    }
+  };
+  for (size_t i = 0; i < P.len(); ++i) {
+    if (P.kind[i] != KIND_FLUID) continue;
+    eat(P.x[i].x);
+    eat(P.x[i].y);
+    eat(P.x[i].z);
+    eat(P.v[i].x);
+    eat(P.v[i].y);
+    eat(P.v[i].z);
+  }
+  return hsh;
+}
+
+}  // namespace ora
+
+// ===========================================================================
+// C interface for the Python test harness (ctypes).
+// ===========================================================================
+using namespace ora;
+
+extern "C" {
+
+typedef struct {
+  int32_t dim;
+  double rest_density, mass, h;
+  double gamma, sound_speed;
+  double visc_alpha, visc_beta, visc_eps;
+  double xsph_eps;
+  double gravity[3];
+  int32_t adapt_sound_speed;
+} OraParams;
+
+typedef struct {
+  double total_mass;
+  double momentum[3];
+  double kinetic_energy;
+  double potential_energy;  // sum m g z with the given g magnitude
+  double mean_density_error, max_density_error;
+  double v_max, a_max;
+  uint64_t hash;
+} OraDiag;
+
+// --- kernels and closed forms ------------------------------------------------
+double ora_w(double r, double h, int32_t dim) { return w(r, h, dim); }
+double ora_dwdr(double r, double h, int32_t dim) { return dwdr(r, h, dim); }
+void ora_grad_w(const double* rij, double h, int32_t dim, double* out) {
+  V3 g = grad_w(v3(rij[0], rij[1], rij[2]), h, dim);
+  out[0] = g.x;
+  out[1] = g.y;
+  out[2] = g.z;
+}
+double ora_tait_pressure(double rho, double rho0, double c, double gamma) { return tait_pressure(rho, rho0, c, gamma); }
+double ora_ensure_sound_speed(double c, double vmax) { return ensure_sound_speed(c, vmax); }
+double ora_pi_ij(double alpha, double beta, double eps, double h, double c, const double* xi, const double* xj,
+                 const double* vi, const double* vj, double rho_i, double rho_j) {
+  return pi_ij(Visc{alpha, beta, eps}, h, c, v3(xi[0], xi[1], xi[2]), v3(xj[0], xj[1], xj[2]), v3(vi[0], vi[1], vi[2]),
+               v3(vj[0], vj[1], vj[2]), rho_i, rho_j);
+}
+int32_t ora_floor_i32(double x) { return floor_i32(x); }
+double ora_mass_from_spacing(double rho0, double dx, int32_t dim) { return mass_from_spacing(rho0, dx, dim); }
+double ora_mass_calibrated_to_rest(double rho0, double dx, double h, int32_t dim) {
+  return mass_calibrated_to_rest(rho0, dx, h, dim);
+}
+
+// --- RNG ---------------------------------------------------------------------
+void* ora_rng_new(uint64_t seed) { return new Rng(seed); }
+void ora_rng_free(void* r) { delete (Rng*)r; }
+void ora_rng_set_state(void* r, const uint64_t* st) {
+  for (int k = 0; k < 4; ++k) ((Rng*)r)->s[k] = st[k];
+}
+uint64_t ora_splitmix64(uint64_t* state) { return Rng::splitmix64(*state); }
+uint64_t ora_rng_next_u64(void* r) { return ((Rng*)r)->next_u64(); }
+double ora_rng_uniform(void* r, double lo, double hi) { return ((Rng*)r)->uniform(lo, hi); }
+
+// --- particles / scene builders ---------------------------------------------
+void* ora_parts_new(int32_t dim, double rho0, double mass, double h) {
+  Particles* P = new Particles();
+  P->dim = dim;
+  P->rest_density = rho0;
+  P->mass = mass;
+  P->h = h;
+  return P;
+}
+void ora_parts_free(void* p) { delete (Particles*)p; }
+uint64_t ora_parts_len(void* p) { return ((Particles*)p)->len(); }
+void ora_parts_push_fluid(void* p, const double* x, const double* v) {
+  ((Particles*)p)->push_fluid(v3(x[0], x[1], x[2]), v3(v[0], v[1], v[2]));
+}
+void ora_parts_push_boundary(void* p, const double* x, double psi, const double* n) {
+  ((Particles*)p)->push_boundary_oriented(v3(x[0], x[1], x[2]), psi, v3(n[0], n[1], n[2]));
+}
+void ora_fill_box(void* p, const double* origin, uint64_t nx, uint64_t ny, uint64_t nz, double dx, const double* v0) {
+  fill_box(*(Particles*)p, v3(origin[0], origin[1], origin[2]), nx, ny, nz, dx, v3(v0[0], v0[1], v0[2]));
+}
+void ora_fill_box_boundary(void* p, const double* mn, const double* mx, double dx, uint64_t layers) {
+  fill_box_boundary(*(Particles*)p, v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), dx, layers);
+}
+void ora_recomputed_akinci_psi(void* p, int32_t strict) { recomputed_akinci_psi(*(Particles*)p, strict != 0); }
+// Jitter fluid positions: per fluid particle in index order, per component x, y, z.
+void ora_parts_jitter(void* p, uint64_t seed, double amp) {
+  Particles& P = *(Particles*)p;
+  Rng r(seed);
+  for (size_t i = 0; i < P.len(); ++i) {
+    if (P.kind[i] != KIND_FLUID) continue;
+    P.x[i].x += r.uniform(-amp, amp);
+    P.x[i].y += r.uniform(-amp, amp);
+    P.x[i].z += r.uniform(-amp, amp);
+  }
+}
+void ora_parts_get(void* p, double* x, double* v, uint8_t* kind, double* psi, double* n_out) {
+  Particles& P = *(Particles*)p;
+  for (size_t i = 0; i < P.len(); ++i) {
+    if (x) {
+      x[3 * i] = P.x[i].x;
+      x[3 * i + 1] = P.x[i].y;
+      x[3 * i + 2] = P.x[i].z;
+    }
+    if (v) {
+      v[3 * i] = P.v[i].x;
+      v[3 * i + 1] = P.v[i].y;
+      v[3 * i + 2] = P.v[i].z;
+    }
+    if (kind) kind[i] = P.kind[i];
+    if (psi) psi[i] = P.psi[i];
+    if (n_out) {
+      n_out[3 * i] = P.n_out[i].x;
+      n_out[3 * i + 1] = P.n_out[i].y;
+      n_out[3 * i + 2] = P.n_out[i].z;
+    }
+  }
+}
+
+// --- world -------------------------------------------------------------------
+void* ora_world_new(const OraParams* prm, uint64_t n, const double* x, const double* v, const uint8_t* kind,
+                    const double* psi, const double* n_out, int32_t mode, int32_t threads) {
+  World* W = new World();
+  W->P.dim = prm->dim;
+  W->P.rest_density = prm->rest_density;
+  W->P.mass = prm->mass;
+  W->P.h = prm->h;
+  for (size_t i = 0; i < n; ++i) {
+    V3 px = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    if (kind[i] == KIND_FLUID) {
+      W->P.push_fluid(px, v ? v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]) : v3(0, 0, 0));
+    } else {
+      W->P.push_boundary_oriented(px, psi ? psi[i] : 0.0,
+                                  n_out ? v3(n_out[3 * i], n_out[3 * i + 1], n_out[3 * i + 2]) : v3(0, 0, 0));
+    }
+  }
+  W->params.gamma = prm->gamma;
+  W->params.sound_speed = prm->sound_speed;
+  W->params.visc = Visc{prm->visc_alpha, prm->visc_beta, prm->visc_eps};
+  W->params.xsph_eps = prm->xsph_eps;
+  W->params.gravity = v3(prm->gravity[0], prm->gravity[1], prm->gravity[2]);
+  W->params.adapt_sound_speed = prm->adapt_sound_speed != 0;
+  W->mode = mode;
+  W->threads = threads < 1 ? 1 : threads;
+  W->init();
+  return W;
+}
+void ora_world_free(void* w) { delete (World*)w; }
+void ora_world_set_threads(void* w, int32_t threads) { ((World*)w)->threads = threads < 1 ? 1 : threads; }
+void ora_world_step(void* w, double dt, uint64_t nsteps) {
+  for (uint64_t s = 0; s < nsteps; ++s) ((World*)w)->step(dt);
+}
+void ora_world_rebuild_grid(void* w) { ((World*)w)->rebuild_grid(); }
+void ora_world_compute_density(void* w) { ((World*)w)->compute_density(); }
+void ora_world_compute_pressure(void* w) { ((World*)w)->compute_pressure(); }
+void ora_world_compute_forces(void* w) { ((World*)w)->compute_forces(); }
+void ora_world_apply_xsph(void* w) { ((World*)w)->apply_xsph(); }
+double ora_world_time(void* w) { return ((World*)w)->t; }
+double ora_world_sound_speed(void* w) { return ((World*)w)->params.sound_speed; }
+void ora_world_set_sound_speed(void* w, double c) { ((World*)w)->params.sound_speed = c; }
+uint64_t ora_world_len(void* w) { return ((World*)w)->P.len(); }
+
+// Copy state out (any pointer may be null): x, v, a, boundary_force as 3n; rho, p as n.
+void ora_world_get_state(void* w, double* x, double* v, double* a, double* rho, double* p, double* bforce) {
+  World& W = *(World*)w;
+  for (size_t i = 0; i < W.P.len(); ++i) {
+    const V3* src[4] = {&W.P.x[i], &W.P.v[i], &W.P.a[i], &W.boundary_force[i]};
+    double* dst[4] = {x, v, a, bforce};
+    for (int k = 0; k < 4; ++k)
+      if (dst[k]) {
+        dst[k][3 * i] = src[k]->x;
+        dst[k][3 * i + 1] = src[k]->y;
+        dst[k][3 * i + 2] = src[k]->z;
+      }
+    if (rho) rho[i] = W.P.rho[i];
+    if (p) p[i] = W.P.p[i];
+  }
+}
+// Overwrite positions / velocities (either may be null).
+void ora_world_set_state(void* w, const double* x, const double* v) {
+  World& W = *(World*)w;
+  for (size_t i = 0; i < W.P.len(); ++i) {
+    if (x) W.P.x[i] = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    if (v) W.P.v[i] = v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  }
+}
+void ora_world_set_rho_p(void* w, const double* rho, const double* p) {
+  World& W = *(World*)w;
+  for (size_t i = 0; i < W.P.len(); ++i) {
+    if (rho) W.P.rho[i] = rho[i];
+    if (p) W.P.p[i] = p[i];
+  }
+}
+
+// Grid access after rebuild_grid: keys (3 i32 per particle), order (n),
+// occupied cell count, occupied keys (3 i32 each), ranges (2 u64 each).
+uint64_t ora_world_grid_cells(void* w) { return ((World*)w)->grid.occupied.size(); }
+void ora_world_get_grid(void* w, int32_t* keys, uint64_t* order, int32_t* occupied, uint64_t* ranges) {
+  World& W = *(World*)w;
+  const Grid& G = W.grid;
+  for (size_t i = 0; i < G.keys.size(); ++i) {
+    if (keys) {
+      keys[3 * i] = G.keys[i].ix;
+      keys[3 * i + 1] = G.keys[i].iy;
+      keys[3 * i + 2] = G.keys[i].iz;
+    }
+    if (order) order[i] = G.order[i];
+  }
+  for (size_t c = 0; c < G.occupied.size(); ++c) {
+    if (occupied) {
+      occupied[3 * c] = G.occupied[c].ix;
+      occupied[3 * c + 1] = G.occupied[c].iy;
+      occupied[3 * c + 2] = G.occupied[c].iz;
+    }
+    if (ranges) {
+      ranges[2 * c] = G.ranges[c].first;
+      ranges[2 * c + 1] = G.ranges[c].second;
+    }
+  }
+}
+// Reference candidate list of particle i (grid.rs:122-140). Returns the full
+// length; writes min(len, cap) entries.
+uint64_t ora_world_neighbours(void* w, uint64_t i, uint64_t* out, uint64_t cap) {
+  World& W = *(World*)w;
+  std::vector<uint64_t> tmp;
+  W.grid.neighbours((size_t)i, tmp);
+  for (size_t k = 0; k < tmp.size() && k < cap; ++k) out[k] = tmp[k];
+  return tmp.size();
+}
+
+void ora_world_diagnostics(void* w, double g, OraDiag* d) {  // diagnostics.rs:19-102, wcsph.rs:350-369
+  World& W = *(World*)w;
+  const Particles& P = W.P;
+  d->total_mass = P.mass * (double)P.fluid_count();
+  V3 mom = v3(0, 0, 0);
+  double ke = 0.0, pe = 0.0, s = 0.0, mx = 0.0;
+  size_t nf = 0;
+  for (size_t i = 0; i < P.len(); ++i) {
+    if (P.kind[i] != KIND_FLUID) continue;
+    mom = add(mom, mul(P.v[i], P.mass));
+    ke += 0.5 * P.mass * norm2(P.v[i]);
+    pe += P.mass * g * P.x[i].z;
+    double e = std::fabs(P.rho[i] - P.rest_density) / P.rest_density;
+    s += e;
+    if (e > mx) mx = e;
+    ++nf;
+  }
+  d->momentum[0] = mom.x;
+  d->momentum[1] = mom.y;
+  d->momentum[2] = mom.z;
+  d->kinetic_energy = ke;
+  d->potential_energy = pe;
+  d->mean_density_error = nf ? s / (double)nf : 0.0;
+  d->max_density_error = mx;
+  d->v_max = W.max_speed();
+  d->a_max = W.a_max();
+  d->hash = particles_hash64(P);
+}
+
+uint64_t ora_fnv1a_f64(const double* vals, uint64_t n) {
+  uint64_t hsh = 0xcbf29ce484222325ULL;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t b;
+    std::memcpy(&b, &vals[i], 8);
+    for (int k = 0; k < 8; ++k) {
+      hsh ^= (b >> (8 * k)) & 0xffULL;
+      hsh *= 0x100000001b3ULL;
+    }
+  }
+  return hsh;
+}
+uint64_t ora_fnv1a_bytes(const uint8_t* bytes, uint64_t n) {
+  uint64_t hsh = 0xcbf29ce484222325ULL;
+  for (uint64_t i = 0; i < n; ++i) {
+    hsh ^= bytes[i];
+    hsh *= 0x100000001b3ULL;
+  }
+  return hsh;
+}
+
+}  // extern "C"
