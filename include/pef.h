@@ -1,0 +1,245 @@
+/*
+ * pef.h -- C ABI of the MI355X (gfx950) WCSPH step.
+ *
+ * Drop-in boundary for pe-fluid's weakly compressible SPH world. Each entry
+ * point replaces one item of the reference's Rust API; the reference item is
+ * cited (file:line, relative to the reference checkout) next to it.
+ *
+ * Conventions follow crates/pe-ffi (error codes error.rs:4-16, one owner thread
+ * per handle lib.rs:3-13, caller-allocated host buffers docs/FFI.md:14-15,
+ * size-probe getters lib.rs:50-66, no exception crosses the boundary
+ * lib.rs:68-80):
+ *   - every function returns an int32 status (PEF_OK on success);
+ *   - a PefWorld is used from one thread at a time; distinct worlds may run on
+ *     distinct threads; each world owns one HIP stream on the device that was
+ *     current when it was created;
+ *   - the library owns device memory and the handle; the caller owns every host
+ *     buffer it passes; getters copy out and never retain caller pointers;
+ *   - arrays of vectors are flat, 3 doubles per particle (x0,y0,z0,x1,...), in
+ *     the caller's original particle order (the reference's index i);
+ *   - non-finite inputs and dt <= 0 are rejected with PEF_ERR_ARG;
+ *   - device-side faults of the simulation (a particle leaving the device grid,
+ *     a neighbour list above its capacity) are sticky and reported by the call
+ *     that detects them.
+ */
+#ifndef PEF_H
+#define PEF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (crates/pe-ffi/src/error.rs:4-16, same integers). */
+#define PEF_OK 0
+#define PEF_ERR_NULL 1          /* null handle or pointer where one is required */
+#define PEF_ERR_BOUNDS 2        /* index or buffer length out of range */
+#define PEF_ERR_ARG 3           /* invalid argument: NaN/Inf, dt <= 0, bad enum */
+#define PEF_ERR_PANIC 4         /* internal error or HIP runtime failure */
+#define PEF_ERR_UNSUPPORTED 6   /* feature of the reference not on this path */
+#define PEF_ERR_GRID 7          /* a particle left the device cell grid (sticky) */
+#define PEF_ERR_NEIGHBOURS 8    /* a neighbour list exceeded its capacity (sticky) */
+#define PEF_ERR_DEVICE 9        /* no usable HIP device */
+
+/* Particle kinds (crates/pe-fluid/src/particles.rs:11-17). */
+#define PEF_FLUID 0
+#define PEF_BOUNDARY 1
+
+/* Kernel families (crates/pe-fluid/src/kernels.rs:19-28). Only the cubic
+ * spline, the WCSPH default (wcsph.rs:45), is implemented. */
+#define PEF_KERNEL_CUBIC_SPLINE 0
+
+/* Sub-phases of WcsphWorld::step (wcsph.rs:102-323). */
+#define PEF_PHASE_REBUILD_GRID 0    /* wcsph.rs:102-114 */
+#define PEF_PHASE_DENSITY 1         /* wcsph.rs:117-154 */
+#define PEF_PHASE_PRESSURE 2        /* wcsph.rs:157-178 */
+#define PEF_PHASE_FORCES 3          /* wcsph.rs:181-280 */
+#define PEF_PHASE_XSPH 4            /* wcsph.rs:283-323 */
+
+/*
+ * WcsphParams (wcsph.rs:22-39) flattened with the Particles scalars
+ * (particles.rs:20-28). pef_default_params() fills WcsphParams::water
+ * (wcsph.rs:43-54): cubic spline, gamma 7, alpha 0.01, beta 0, eps 0.01,
+ * XSPH 0.25, adaptive sound speed on, no surface tension.
+ */
+typedef struct PefParams {
+  int32_t dim;                 /* 2 or 3 (lib.rs Dim) */
+  int32_t kernel;              /* PEF_KERNEL_CUBIC_SPLINE */
+  double rest_density;         /* rho0 [kg/m^3] */
+  double mass;                 /* fluid particle mass [kg] */
+  double h;                    /* smoothing length [m] */
+  double gamma;                /* Tait exponent (eos.rs:32-39) */
+  double sound_speed;          /* c0 [m/s]; updated by the step when adaptive */
+  double visc_alpha;           /* MonaghanViscosity (viscosity.rs:30-47) */
+  double visc_beta;
+  double visc_eps;
+  double xsph_epsilon;         /* Xsph (viscosity.rs:77-86); 0 disables */
+  double gravity[3];           /* [m/s^2] */
+  int32_t adapt_sound_speed;   /* c >= 10 v_max each step (wcsph.rs:159-169) */
+  double surface_tension_sigma;/* must be 0 (surface tension is not on this path) */
+} PefParams;
+
+/* Particle arrays for creation (particles.rs:20-48), original order. */
+typedef struct PefParticles {
+  uint64_t n;
+  const double* x;       /* 3n positions [m] (required) */
+  const double* v;       /* 3n velocities [m/s] (NULL: zero) */
+  const uint8_t* kind;   /* n kinds, PEF_FLUID / PEF_BOUNDARY (required) */
+  const double* psi;     /* n Akinci volumes (NULL: zero) */
+  const double* n_out;   /* 3n outward normals of one-sided samples (NULL: zero) */
+} PefParticles;
+
+/* Device-layout options. Zero-initialised fields take the defaults. */
+typedef struct PefOptions {
+  uint32_t neighbour_capacity; /* list slots per particle (default 384) */
+  double skin;                 /* Verlet skin as a fraction of 2h (default 0.08) */
+  uint32_t grid_margin_cells;  /* empty cells kept around the initial bounding box (default 4) */
+  int32_t boundary_force;      /* compute the Akinci reactions each step (default 1; -1 = off) */
+  int32_t use_graph;           /* replay captured HIP graphs for multi-step calls (default 1; -1 = off) */
+  uint32_t block_size;         /* threads per workgroup of the pair kernels (0 = tuned default) */
+} PefOptions;
+
+/* Global conservation and error diagnostics, reduced in FP64 on the device
+ * (diagnostics.rs:19-102, wcsph.rs:350-369). */
+typedef struct PefDiagnostics {
+  double total_mass;           /* diagnostics.rs:36-38 */
+  double momentum[3];          /* diagnostics.rs:41-49 */
+  double kinetic_energy;       /* diagnostics.rs:52-60 */
+  double potential_energy;     /* sum m g z with g = |gravity| (diagnostics.rs:63-71) */
+  double mean_density_error;   /* diagnostics.rs:74-88 */
+  double max_density_error;    /* diagnostics.rs:91-102 */
+  double v_max;                /* wcsph.rs:350-358 */
+  double a_max;                /* wcsph.rs:361-369 */
+  uint64_t hash;               /* FNV-1a of fluid (x, v) f64 bits (diagnostics.rs:19-33) */
+  uint64_t fluid_count;
+} PefDiagnostics;
+
+typedef struct PefWorld PefWorld;
+
+/* Library identification. */
+const char* pef_version_string(void);
+
+/* WcsphParams::water(rho0, c0, g) (wcsph.rs:43-54) plus Particles scalars. */
+int32_t pef_default_params(double rest_density, double sound_speed, const double gravity[3],
+                           PefParams* out);
+
+/* WcsphWorld::new (wcsph.rs:82-99). Uploads the particles and allocates the
+ * device state; `options` may be NULL. */
+int32_t pef_world_create(const PefParams* params, const PefParticles* particles,
+                         const PefOptions* options, PefWorld** out);
+
+/* Many independent scenes in one world: scene k has its own params and
+ * particles; every call then advances or reads all scenes together, and the
+ * particle arrays of all scenes are concatenated scene after scene. No
+ * reference counterpart: the reference has one scene per WcsphWorld. */
+int32_t pef_batch_create(uint32_t n_scenes, const PefParams* params, const PefParticles* particles,
+                         const PefOptions* options, PefWorld** out);
+
+/* Drop (pe-ffi lib.rs:84-100 convention). NULL is a no-op. */
+void pef_world_destroy(PefWorld* world);
+
+/* WcsphWorld::step(dt) repeated nsteps times (wcsph.rs:326-347). Synchronous. */
+int32_t pef_world_step(PefWorld* world, double dt, uint32_t nsteps);
+
+/* One sub-phase of the step (PEF_PHASE_*), for callers that drive
+ * rebuild_grid / compute_density / compute_pressure / compute_forces /
+ * apply_xsph themselves (tests/fluid_analytic.rs:154-158). FORCES computes
+ * accelerations only; it does not kick or drift. */
+int32_t pef_world_phase(PefWorld* world, int32_t phase);
+
+/* Counts. */
+int32_t pef_world_len(const PefWorld* world, uint64_t* n);
+int32_t pef_world_scene_count(const PefWorld* world, uint32_t* n_scenes);
+/* First particle and particle count of scene k in the concatenated arrays. */
+int32_t pef_world_scene_range(const PefWorld* world, uint32_t scene, uint64_t* first, uint64_t* count);
+
+/* Simulation time (wcsph.rs:65-66). */
+int32_t pef_world_time(const PefWorld* world, double* t);
+
+/* Params of scene k, including the current adapted sound speed
+ * (wcsph.rs:168 writes it back). set_params mutates between steps
+ * (tests/coupling_oracles.rs:73-74); dim, h, mass, rest_density are fixed. */
+int32_t pef_world_get_params(PefWorld* world, uint32_t scene, PefParams* out);
+int32_t pef_world_set_params(PefWorld* world, uint32_t scene, const PefParams* params);
+
+/* Particle state getters, original order, `len` = number of doubles the buffer
+ * holds (3n for vectors, n for scalars); returns PEF_ERR_BOUNDS if too small.
+ * particles.rs:29-47 fields x, v, a, rho, p; wcsph.rs:77-78 boundary_force. */
+int32_t pef_world_get_positions(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_velocities(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_accelerations(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_densities(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_pressures(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_boundary_force(PefWorld* world, double* out, uint64_t len);
+
+/* Setters used by coupling callers between steps (coupling.rs:289-305 writes
+ * boundary x, v, n_out each step); any pointer may be NULL; arrays are 3n or n
+ * in original order. Densities are recomputed at the next step. */
+int32_t pef_world_set_state(PefWorld* world, const double* x, const double* v, const double* psi,
+                            const double* n_out);
+
+/* The reference's uniform grid (grid.rs:31-45) over the current positions:
+ * keys (3n int32, per original index), order (n), occupied (3m int32),
+ * ranges (2m, [start, end) into order). The call returns m in *n_cells; pass
+ * cap_cells < m (or NULL arrays) to probe m. */
+int32_t pef_world_get_grid(PefWorld* world, int32_t* keys, uint64_t* order, int32_t* occupied,
+                           uint64_t* ranges, uint64_t cap_cells, uint64_t* n_cells);
+
+/* UniformGrid::neighbours(i) (grid.rs:122-140): the unfiltered 27-cell (9 in
+ * 2-D) candidate list of particle i in the reference's order, self included.
+ * *out_len receives the length; PEF_ERR_BOUNDS if cap is too small. */
+int32_t pef_world_get_neighbours(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap,
+                                 uint64_t* out_len);
+
+/* The device's own neighbour list of particle i, filtered to r < 2h, in
+ * original indices and in the reference's candidate order. The density, force
+ * and XSPH kernels sum exactly these pairs. */
+int32_t pef_world_get_neighbour_list(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap,
+                                     uint64_t* out_len);
+
+/* FP64 device reductions (diagnostics.rs:19-102) for scene k. */
+int32_t pef_world_diagnostics(PefWorld* world, uint32_t scene, PefDiagnostics* out);
+
+/* Per-kernel timing of the last pef_world_step call, measured with HIP events
+ * on the world's stream when profiling is on. names/ms arrays of length cap;
+ * *n receives the number of kernels. */
+int32_t pef_world_set_profiling(PefWorld* world, int32_t on);
+int32_t pef_world_kernel_times(PefWorld* world, const char** names, double* ms, uint32_t cap,
+                               uint32_t* n);
+/* Mean neighbour-list occupancy (entries per particle within the list radius)
+ * and number of list rebuilds since creation. */
+int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds);
+
+/* ---- scene construction (particles.rs:164-347) ---------------------------- */
+
+/* mass_from_spacing (particles.rs:166-171). */
+int32_t pef_mass_from_spacing(double rest_density, double dx, int32_t dim, double* out);
+/* mass_calibrated_to_rest (particles.rs:178-215), cubic spline. */
+int32_t pef_mass_calibrated_to_rest(double rest_density, double dx, double h, int32_t dim,
+                                    double* out);
+/* fill_box (particles.rs:220-245): writes the lattice positions (3 per
+ * particle, iz -> iy -> ix order) into out; *n receives the count. */
+int32_t pef_fill_box(int32_t dim, const double origin[3], uint64_t nx, uint64_t ny, uint64_t nz,
+                     double dx, double* out, uint64_t cap, uint64_t* n);
+/* fill_box_boundary (particles.rs:286-347): shell sample positions; the
+ * provisional psi is dx^dim for each. */
+int32_t pef_fill_box_boundary(int32_t dim, const double mn[3], const double mx[3], double dx,
+                              uint64_t layers, double* out, uint64_t cap, uint64_t* n);
+/* recomputed_akinci_psi (particles.rs:252-279) on the GPU: psi of every
+ * boundary particle from its boundary neighbours, summed in ascending index
+ * order in FP64 exactly as the reference does. psi is read and written. */
+int32_t pef_recomputed_akinci_psi(int32_t dim, double h, uint64_t n, const double* x,
+                                  const uint8_t* kind, double* psi);
+
+/* Jitter fluid positions as the dam-break configs prescribe: per fluid
+ * particle in index order, per component x, y, z, add uniform(-amp, amp) drawn
+ * from Xoshiro256StarStar::seed(seed) (crates/pe-math/src/rng.rs:35, 70). */
+int32_t pef_jitter_positions(uint64_t seed, double amp, uint64_t n, const uint8_t* kind, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEF_H */

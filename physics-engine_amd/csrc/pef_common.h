@@ -1,0 +1,163 @@
+// Shared device/host definitions of the MI355X WCSPH step.
+//
+// Device data layout (all arrays indexed by "slot", the particle's position in
+// the current cell-sorted order; scenes occupy fixed contiguous slot ranges):
+//   X, Y, Z   f64  positions: the long-run position accumulator
+//   A         f32x4 (x, y, z, w): position relative to the corner of the cell
+//             the particle was binned into at the last list build; w = +m for
+//             fluid, -rho0*psi for boundary (sign bit = boundary)
+//   A2        f32x4 same positions, w = rho for fluid (written by density)
+//   B         f32x4 (vx, vy, vz, -) velocities
+//   B2        f32x4 (vx, vy, vz, P) velocities after XSPH, P = p / rho^2
+//   ID        u32  original particle index (global across scenes)
+//   META      u32  bit0 boundary, bit1 oriented boundary, bits 8.. scene index
+//   CELL      u32  global id of the cell at the last list build
+//   DISP      f32  displacement bound since the last list build
+//   LIST      u32  neighbour entries, ELL by 64-slot chunk:
+//             LIST[(chunk * cap + k) * 64 + lane]; entry = slot | cell-offset
+//             code << 26 (2-bit signed dx, dy, dz of the neighbour's build cell
+//             relative to the particle's build cell)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pef {
+
+constexpr int kWave = 64;
+constexpr uint32_t kSlotBits = 26;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+constexpr uint64_t kMaxSlots = (1ull << kSlotBits);
+
+constexpr uint32_t kMetaBoundary = 1u;
+constexpr uint32_t kMetaOriented = 2u;
+constexpr uint32_t kMetaSceneShift = 8u;
+
+// Sticky error flags in Ctl::flags.
+constexpr uint32_t kFlagGrid = 1u;
+constexpr uint32_t kFlagNeighbours = 2u;
+
+// Per-scene constants (device copy).
+struct SceneDev {
+  // internal (list) grid: cell = 2h (1 + skin)
+  double cell;
+  double inv_cell;
+  int32_t ix0, iy0, iz0;
+  int32_t nx, ny, nz;
+  uint32_t cell_base;
+  uint32_t slot_first, slot_count;
+  uint32_t id_first;
+  int32_t dim;
+  int32_t adapt;
+  int32_t gamma_int;  // gamma when it is an integer in [1, 16], else 0
+  int32_t oriented;   // scene holds one-sided boundary samples
+  float cell_f;
+  float h, inv_h, mass, rho0, inv_rho0;
+  float alpha;       // cubic spline normalisation
+  float r2_list;     // (2h (1 + skin))^2
+  float r2_sup;      // (2h)^2
+  float visc_alpha, visc_beta, visc_eps_h2;
+  float xsph_eps;
+  float gamma;
+  float g[3];
+  float skin_half;   // half the skin length [m]
+  double h_d;
+  double rho0_d;
+  double gamma_d;
+};
+
+// Mutable per-scene state on the device.
+struct SceneState {
+  double c;            // sound speed in use (adapted, monotone)
+  double B;            // Tait stiffness rho0 c^2 / gamma
+  uint32_t vmax2_bits; // max |v|^2 of fluid, float bits (non-negative floats order as uints)
+  uint32_t pad;
+};
+
+// Step control block on the device.
+struct Ctl {
+  double dt;
+  uint32_t need_rebuild;   // set by the force kernel when a displacement exceeds skin/2
+  uint32_t force_rebuild;  // set by the host
+  uint32_t rebuilding;     // latched for the current step by k_decide
+  uint32_t flags;          // sticky kFlag* bits
+  uint32_t max_count;      // largest list length seen at the last build
+  uint32_t pad;
+  unsigned long long list_entries;  // total list entries at the last build
+  unsigned long long rebuilds;
+};
+
+// FP64 diagnostics partials (one per block) and result.
+struct DiagAcc {
+  double mom[3];
+  double ke;
+  double pe;
+  double derr_sum;
+  double derr_max;
+  double v2max;
+  double a2max;
+  double nfluid;
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+// floor toward -inf with the reference's clamp and non-finite -> 0
+// (crates/pe-fluid/src/grid.rs:144-157).
+__device__ __host__ inline int32_t floor_i32(double x) {
+  if (!(x == x) || x == __builtin_inf() || x == -__builtin_inf()) return 0;
+  const double lo = -2147483648.0 + 1.0;
+  const double hi = 2147483647.0 - 1.0;
+  double c = x < lo ? lo : (x > hi ? hi : x);
+  int32_t t = (int32_t)c;
+  if ((double)t > c) return t - 1;
+  return t;
+}
+
+__device__ inline uint32_t scene_of(uint32_t meta) { return meta >> kMetaSceneShift; }
+__device__ inline bool is_boundary_w(float w) { return (__float_as_uint(w) >> 31) != 0u; }
+
+// Cubic spline W and dW/dr in FP32 (kernels.rs:79-116); q < 2 is the caller's
+// guarantee (pairs are filtered on r^2 < (2h)^2 first).
+__device__ inline float cubic_w(float q, float alpha) {
+  float t = 2.0f - q;
+  float inner = 1.0f - q * q * (1.5f - 0.75f * q);
+  float outer = 0.25f * t * t * t;
+  return alpha * (q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f));
+}
+__device__ inline float cubic_dwdr(float q, float alpha_over_h) {
+  float t = 2.0f - q;
+  float inner = q * (-3.0f + 2.25f * q);
+  float outer = -0.75f * t * t;
+  return alpha_over_h * (q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f));
+}
+
+// (1 + d)^n - 1 for integer n >= 1 by the binomial expansion in Horner form;
+// accurate for small d where the direct power cancels.
+__device__ inline float pow1p_m1_int(float d, int n) {
+  float acc = 1.0f;  // C(n, n)
+  float coef = 1.0f;
+  for (int k = n - 1; k >= 1; --k) {
+    coef = coef * (float)(k + 1) / (float)(n - k);  // C(n, k) from C(n, k+1)
+    acc = coef + d * acc;
+  }
+  return d * acc;
+}
+
+// Tait pressure p = B ((rho/rho0)^gamma - 1) (eos.rs:65-69) in FP32.
+__device__ inline float tait_p(float rho, const SceneDev& S, float B) {
+  float d = (rho - S.rho0) * S.inv_rho0;
+  float pm1 = (S.gamma_int > 0) ? pow1p_m1_int(d, S.gamma_int) : expm1f(S.gamma * log1pf(d));
+  return B * pm1;
+}
+
+// XCD-aware block index: the hardware deals workgroups to the 8 XCDs round
+// robin; remap so each XCD walks a contiguous range of blocks (neighbouring
+// chunks share neighbour data in that XCD's L2).
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t G) {
+  uint32_t x = b & 7u, i = b >> 3;
+  uint32_t q = G >> 3, r = G & 7u;
+  return x * q + (x < r ? x : r) + i;
+}
+
+}  // namespace pef

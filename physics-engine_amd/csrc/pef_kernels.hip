@@ -1,0 +1,1046 @@
+// HIP kernels of the WCSPH step for gfx950 (CDNA4, wave64).
+//
+// Kernel map (reference function -> kernel):
+//   UniformGrid::key_of + rebuild (grid.rs:62-107)  -> k_cell_count, k_scan_*,
+//                                                      k_scatter, k_finalize
+//   UniformGrid::neighbours (grid.rs:122-140)        -> k_nlist (filtered list)
+//   WcsphWorld::compute_density (wcsph.rs:117-154)   -> k_density
+//   compute_pressure (wcsph.rs:157-178)              -> k_vmax + k_eos
+//   compute_forces + kick/drift (wcsph.rs:181-342)   -> k_force
+//   apply_xsph (wcsph.rs:283-323)                    -> k_xsph
+//   diagnostics (diagnostics.rs:19-102)              -> k_diag_partial/final
+#include "pef_common.h"
+#include "pef_launch.h"
+
+namespace pef {
+
+// ---------------------------------------------------------------------------
+// small utilities
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_decide(Ctl* ctl, SceneState* st, uint32_t n_scenes) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    uint32_t r = (ctl->need_rebuild | ctl->force_rebuild) ? 1u : 0u;
+    ctl->rebuilding = r;
+    ctl->need_rebuild = 0u;
+    ctl->force_rebuild = 0u;
+    if (r) ctl->rebuilds += 1ull;
+  }
+  (void)st;
+  (void)n_scenes;
+}
+
+// ---------------------------------------------------------------------------
+// grid build: keys and per-cell counts (grid.rs:62-71, 81-84)
+// ---------------------------------------------------------------------------
+__global__ void k_cell_count(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
+                             const uint32_t* __restrict__ meta, const double* __restrict__ X,
+                             const double* __restrict__ Y, const double* __restrict__ Z,
+                             uint32_t* __restrict__ cell_of, uint32_t* __restrict__ rank_of,
+                             uint32_t* __restrict__ counts, int always) {
+  if (!always && !ctl->rebuilding) return;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const SceneDev& S = sc[scene_of(meta[s])];
+  int32_t kx = floor_i32(X[s] * S.inv_cell);
+  int32_t ky = floor_i32(Y[s] * S.inv_cell);
+  int32_t kz = (S.dim == 2) ? 0 : floor_i32(Z[s] * S.inv_cell);
+  int64_t lx = (int64_t)kx - S.ix0, ly = (int64_t)ky - S.iy0, lz = (int64_t)kz - S.iz0;
+  bool out = lx < 0 || ly < 0 || lz < 0 || lx >= S.nx || ly >= S.ny || lz >= S.nz;
+  if (out) {
+    atomicOr(&ctlw->flags, kFlagGrid);
+    lx = lx < 0 ? 0 : (lx >= S.nx ? S.nx - 1 : lx);
+    ly = ly < 0 ? 0 : (ly >= S.ny ? S.ny - 1 : ly);
+    lz = lz < 0 ? 0 : (lz >= S.nz ? S.nz - 1 : lz);
+  }
+  uint32_t cid = S.cell_base + ((uint32_t)lx * (uint32_t)S.ny + (uint32_t)ly) * (uint32_t)S.nz + (uint32_t)lz;
+  cell_of[s] = cid;
+  rank_of[s] = atomicAdd(&counts[cid], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of cell counts -> cell_start (size C + 1)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kScanThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  uint32_t x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(x, o, kWave);
+    if (lane >= o) x += t;
+  }
+  if (lane == kWave - 1) wsum[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    const int nw = blockDim.x / kWave;
+    uint32_t y = (lane < nw) ? wsum[lane] : 0u;
+    for (int o = 1; o < kWave; o <<= 1) {
+      uint32_t t = __shfl_up(y, o, kWave);
+      if (lane >= o) y += t;
+    }
+    if (lane < nw) wsum[lane] = y;  // inclusive wave totals
+  }
+  __syncthreads();
+  uint32_t before = (wid > 0) ? wsum[wid - 1] : 0u;
+  *total = wsum[blockDim.x / kWave - 1];
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_local(const Ctl* __restrict__ ctl, int always,
+                                                             const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out,
+                                                             uint32_t* __restrict__ block_sums, uint32_t C) {
+  if (!always && !ctl->rebuilding) return;
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    v[k] = (i < C) ? in[i] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan(sum, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    if (i < C) out[i] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(const Ctl* __restrict__ ctl, int always,
+                                                            uint32_t* __restrict__ block_sums, uint32_t nb) {
+  if (!always && !ctl->rebuilding) return;
+  // nb <= kScanTile (checked on the host): one block scans all block sums.
+  uint32_t base = threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    v[k] = (i < nb) ? block_sums[i] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan(sum, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    if (i < nb) block_sums[i] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) block_sums[nb] = total;
+}
+
+__global__ void k_scan_add(const Ctl* __restrict__ ctl, int always, uint32_t* __restrict__ out,
+                           const uint32_t* __restrict__ block_sums, uint32_t C, uint32_t nb) {
+  if (!always && !ctl->rebuilding) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < C) out[i] += block_sums[i / kScanTile];
+  if (i == 0) out[C] = block_sums[nb];
+}
+
+// ---------------------------------------------------------------------------
+// scatter slots into cell buckets (arbitrary order within a cell)
+// ---------------------------------------------------------------------------
+__global__ void k_scatter(uint32_t N, const Ctl* __restrict__ ctl, int always, const uint32_t* __restrict__ cell_of,
+                          const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ cell_start,
+                          uint32_t* __restrict__ perm) {
+  if (!always && !ctl->rebuilding) return;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  perm[cell_start[cell_of[s]] + rank_of[s]] = s;
+}
+
+// Final position of slot s: cell start + number of cell members with a smaller
+// original index (grid.rs:86-87 sorts by (key, index)).
+__device__ inline uint32_t ranked_dst(uint32_t s, uint32_t c, uint32_t my_id, const uint32_t* __restrict__ cell_start,
+                                      const uint32_t* __restrict__ perm, const uint32_t* __restrict__ ID) {
+  uint32_t b = cell_start[c], e = cell_start[c + 1];
+  uint32_t rank = 0;
+  for (uint32_t p = b; p < e; ++p) rank += (ID[perm[p]] < my_id) ? 1u : 0u;
+  (void)s;
+  return b + rank;  // O(cell population); cells hold ~(4 dx/ h)^3 particles
+}
+
+// Permute the whole particle state into the new cell order (alt buffers) and
+// re-express positions relative to the new cell corner.
+__global__ void k_finalize(uint32_t N, const Ctl* __restrict__ ctl, const SceneDev* __restrict__ sc,
+                           const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cell_start,
+                           const uint32_t* __restrict__ perm, State cur, State alt) {
+  if (!ctl->rebuilding) return;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t c = cell_of[s];
+  uint32_t my_id = cur.ID[s];
+  uint32_t dst = ranked_dst(s, c, my_id, cell_start, perm, cur.ID);
+  uint32_t meta = cur.META[s];
+  const SceneDev& S = sc[scene_of(meta)];
+  uint32_t local = c - S.cell_base;
+  uint32_t lz = local % (uint32_t)S.nz;
+  uint32_t t = local / (uint32_t)S.nz;
+  uint32_t ly = t % (uint32_t)S.ny;
+  uint32_t lx = t / (uint32_t)S.ny;
+  double x = cur.X[s], y = cur.Y[s], z = cur.Z[s];
+  double cx = (double)(S.ix0 + (int32_t)lx) * (double)S.cell_f;
+  double cy = (double)(S.iy0 + (int32_t)ly) * (double)S.cell_f;
+  double cz = (double)(S.iz0 + (int32_t)lz) * (double)S.cell_f;
+  float w = (meta & kMetaBoundary) ? cur.A[s].w : S.mass;
+  alt.X[dst] = x;
+  alt.Y[dst] = y;
+  alt.Z[dst] = z;
+  alt.A[dst] = make_float4((float)(x - cx), (float)(y - cy), (float)(z - cz), w);
+  alt.B[dst] = cur.B[s];
+  alt.B2[dst] = cur.B2[s];
+  float4 a2 = cur.A2[s];
+  alt.A2[dst] = make_float4(0.0f, 0.0f, 0.0f, a2.w);  // density and pressure travel with the particle
+  alt.ID[dst] = my_id;
+  alt.META[dst] = meta;
+  alt.CELL[dst] = c;
+  alt.DISP[dst] = 0.0f;
+  if (meta & kMetaOriented) {
+    alt.NX[dst] = cur.NX[s];
+    alt.NY[dst] = cur.NY[s];
+    alt.NZ[dst] = cur.NZ[s];
+  }
+}
+
+__global__ void k_copyback(uint32_t N, const Ctl* __restrict__ ctl, State alt, State cur,
+                           uint32_t* __restrict__ counts, uint32_t C) {
+  if (!ctl->rebuilding) return;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < C) counts[s] = 0u;  // ready for the next rebuild
+  if (s >= N) return;
+  cur.X[s] = alt.X[s];
+  cur.Y[s] = alt.Y[s];
+  cur.Z[s] = alt.Z[s];
+  cur.A[s] = alt.A[s];
+  cur.A2[s] = alt.A2[s];
+  cur.B[s] = alt.B[s];
+  cur.B2[s] = alt.B2[s];
+  cur.ID[s] = alt.ID[s];
+  uint32_t meta = alt.META[s];
+  cur.META[s] = meta;
+  cur.CELL[s] = alt.CELL[s];
+  cur.DISP[s] = 0.0f;
+  if (meta & kMetaOriented) {
+    cur.NX[s] = alt.NX[s];
+    cur.NY[s] = alt.NY[s];
+    cur.NZ[s] = alt.NZ[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour-list build (grid.rs:122-140 candidates, filtered to the list radius)
+// ---------------------------------------------------------------------------
+// One-sided Akinci filter (particles.rs:126-134) in FP64, evaluated like the
+// reference with no contraction.
+__device__ inline bool sees_f64(double bx, double by, double bz, double nx, double ny, double nz, double fx,
+                                double fy, double fz, double h) {
+#pragma clang fp contract(off)
+  double n2 = nx * nx + ny * ny + nz * nz;
+  if (n2 <= 0.0) return true;
+  double eps = 0.15 * h * sqrt(n2 > 0.0 ? n2 : 0.0);
+  double d = (fx - bx) * nx + (fy - by) * ny + (fz - bz) * nz;
+  return d > -eps;
+}
+
+__global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
+                        const uint32_t* __restrict__ cell_start, State cur, uint32_t* __restrict__ LIST,
+                        uint32_t* __restrict__ CNT, uint32_t cap, int with_boundary, int always) {
+  if (!always && !ctl->rebuilding) return;  // uniform across the grid
+  uint32_t G = gridDim.x;
+  uint32_t blk = xcd_block(blockIdx.x, G);
+  uint32_t s = blk * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0;
+  if (s < N) {
+    uint32_t meta = cur.META[s];
+    const SceneDev& S = sc[scene_of(meta)];
+    const bool i_bnd = (meta & kMetaBoundary) != 0u;
+    if (!i_bnd || with_boundary) {
+      uint64_t base = ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
+      float4 ai = cur.A[s];
+      uint32_t c = cur.CELL[s];
+      uint32_t local = c - S.cell_base;
+      int32_t lz = (int32_t)(local % (uint32_t)S.nz);
+      uint32_t t = local / (uint32_t)S.nz;
+      int32_t ly = (int32_t)(t % (uint32_t)S.ny);
+      int32_t lx = (int32_t)(t / (uint32_t)S.ny);
+      const float L = S.cell_f;
+      const float r2l = S.r2_list;
+      const bool i_or = (meta & kMetaOriented) != 0u;
+      const int zlo = (S.dim == 2) ? 0 : -1, zhi = (S.dim == 2) ? 0 : 1;
+      for (int dz = zlo; dz <= zhi; ++dz) {
+        int32_t z = lz + dz;
+        if (z < 0 || z >= S.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          int32_t y = ly + dy;
+          if (y < 0 || y >= S.ny) continue;
+          for (int dx = -1; dx <= 1; ++dx) {
+            int32_t x = lx + dx;
+            if (x < 0 || x >= S.nx) continue;
+            uint32_t cc = S.cell_base + ((uint32_t)x * (uint32_t)S.ny + (uint32_t)y) * (uint32_t)S.nz + (uint32_t)z;
+            uint32_t b = cell_start[cc], e = cell_start[cc + 1];
+            // position of particle i in the frame of cell cc
+            float px = ai.x - (float)dx * L, py = ai.y - (float)dy * L, pz = ai.z - (float)dz * L;
+            uint32_t code = ((uint32_t)(dx & 3) << 26) | ((uint32_t)(dy & 3) << 28) | ((uint32_t)(dz & 3) << 30);
+            for (uint32_t j = b; j < e; ++j) {
+              float4 aj = cur.A[j];
+              float rx = aj.x - px, ry = aj.y - py, rz = aj.z - pz;
+              float r2 = rx * rx + ry * ry + rz * rz;
+              if (!(r2 < r2l) || j == s) continue;
+              const bool j_bnd = is_boundary_w(aj.w);
+              bool ok;
+              if (!i_bnd) {
+                ok = true;
+                if (j_bnd && S.oriented && (cur.META[j] & kMetaOriented))
+                  ok = sees_f64(cur.X[j], cur.Y[j], cur.Z[j], cur.NX[j], cur.NY[j], cur.NZ[j], cur.X[s], cur.Y[s],
+                                cur.Z[s], S.h_d);
+              } else {
+                ok = !j_bnd;
+                if (ok && i_or)
+                  ok = sees_f64(cur.X[s], cur.Y[s], cur.Z[s], cur.NX[s], cur.NY[s], cur.NZ[s], cur.X[j], cur.Y[j],
+                                cur.Z[j], S.h_d);
+              }
+              if (!ok) continue;
+              if (cnt < cap) LIST[base + (uint64_t)cnt * kWave] = j | code;
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    CNT[s] = cnt < cap ? cnt : cap;
+  }
+  // wave max of the counts (all lanes present), one atomic per wave
+  uint32_t mx = cnt;
+  for (int o = 32; o >= 1; o >>= 1) {
+    uint32_t t2 = __shfl_xor(mx, o, kWave);
+    mx = mx > t2 ? mx : t2;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(&ctlw->max_count, mx);
+    if (mx > cap) atomicOr(&ctlw->flags, kFlagNeighbours);
+  }
+}
+
+// offset of the neighbour's build cell relative to the particle's, in metres
+__device__ inline void entry_offset(uint32_t e, float L, float& ox, float& oy, float& oz) {
+  ox = (float)__builtin_amdgcn_sbfe(e, 26, 2) * L;
+  oy = (float)__builtin_amdgcn_sbfe(e, 28, 2) * L;
+  oz = (float)__builtin_amdgcn_sbfe(e, 30, 2) * L;
+}
+
+// ---------------------------------------------------------------------------
+// density summation with a compensated FP32 sum (wcsph.rs:117-154)
+// ---------------------------------------------------------------------------
+__global__ void k_density(uint32_t N, const SceneDev* __restrict__ sc, State cur, const uint32_t* __restrict__ LIST,
+                          const uint32_t* __restrict__ CNT, uint32_t cap) {
+  uint32_t G = gridDim.x;
+  uint32_t blk = xcd_block(blockIdx.x, G);
+  uint32_t s = blk * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  float4 ai = cur.A[s];
+  if (is_boundary_w(ai.w)) {
+    cur.A2[s] = ai;  // boundary keeps w = -rho0 psi; its density is rho0 (wcsph.rs:126-129)
+    return;
+  }
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, alpha = S.alpha;
+  float sum = S.mass * alpha;  // self term m W(0)
+  float comp = 0.0f;
+  uint32_t n = CNT[s];
+  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t e = lp[(uint64_t)k * kWave];
+    float4 aj = cur.A[e & kSlotMask];
+    float ox, oy, oz;
+    entry_offset(e, L, ox, oy, oz);
+    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
+    float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 < r2s) {
+      float r = __builtin_amdgcn_sqrtf(r2);
+      float term = fabsf(aj.w) * cubic_w(r * inv_h, alpha);
+      float y = term - comp;
+      float tt = sum + y;
+      comp = (tt - sum) - y;
+      sum = tt;
+    }
+  }
+  float rho = fmaxf(sum, 1e-12f);
+  cur.A2[s] = make_float4(ai.x, ai.y, ai.z, rho);
+}
+
+// ---------------------------------------------------------------------------
+// XSPH velocity smoothing (wcsph.rs:283-323), Jacobi: reads B, writes B2
+// ---------------------------------------------------------------------------
+__global__ void k_xsph(uint32_t N, const SceneDev* __restrict__ sc, State cur, const uint32_t* __restrict__ LIST,
+                       const uint32_t* __restrict__ CNT, uint32_t cap) {
+  uint32_t G = gridDim.x;
+  uint32_t blk = xcd_block(blockIdx.x, G);
+  uint32_t s = blk * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  float4 ai = cur.A2[s];
+  float4 bi = cur.B[s];
+  const float Pold = cur.B2[s].w;
+  if (is_boundary_w(ai.w)) {
+    cur.B2[s] = make_float4(bi.x, bi.y, bi.z, Pold);
+    return;
+  }
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  if (S.xsph_eps == 0.0f) {
+    cur.B2[s] = make_float4(bi.x, bi.y, bi.z, Pold);
+    return;
+  }
+  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, alpha = S.alpha, m = S.mass;
+  float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+  uint32_t n = CNT[s];
+  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t e = lp[(uint64_t)k * kWave];
+    uint32_t j = e & kSlotMask;
+    float4 aj = cur.A2[j];
+    if (is_boundary_w(aj.w)) continue;
+    float ox, oy, oz;
+    entry_offset(e, L, ox, oy, oz);
+    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
+    float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 < r2s) {
+      float4 bj = cur.B[j];
+      float r = __builtin_amdgcn_sqrtf(r2);
+      float wij = cubic_w(r * inv_h, alpha);
+      float sfac = m * __builtin_amdgcn_rcpf(0.5f * (ai.w + aj.w)) * wij;
+      dvx += (bj.x - bi.x) * sfac;
+      dvy += (bj.y - bi.y) * sfac;
+      dvz += (bj.z - bi.z) * sfac;
+    }
+  }
+  const float eps = S.xsph_eps;
+  float vx = bi.x + dvx * eps, vy = bi.y + dvy * eps, vz = bi.z + dvz * eps;
+  if (S.dim == 2) vz = 0.0f;
+  cur.B2[s] = make_float4(vx, vy, vz, Pold);
+}
+
+// ---------------------------------------------------------------------------
+// sound-speed adaptation (viscosity.rs:89-100, eos.rs:74-81) and Tait EOS
+// ---------------------------------------------------------------------------
+__global__ void k_vmax_zero(SceneState* __restrict__ st, uint32_t n_scenes) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_scenes) st[k].vmax2_bits = 0u;
+}
+
+__global__ void k_vmax(uint32_t N, const float4* __restrict__ V, const uint32_t* __restrict__ META,
+                       SceneState* __restrict__ st) {
+  // No early return: every lane takes part in the shuffles.
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  bool pending = s < N;
+  uint32_t my_scene = 0;
+  float v2 = 0.0f;
+  if (pending) {
+    uint32_t meta = META[s];
+    my_scene = scene_of(meta);
+    if (!(meta & kMetaBoundary)) {
+      float4 v = V[s];
+      v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    }
+  }
+  while (true) {
+    unsigned long long mask = __ballot(pending);
+    if (mask == 0ull) break;
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t lscene = __shfl(my_scene, leader, kWave);
+    float val = (pending && my_scene == lscene) ? v2 : 0.0f;
+    for (int o = 32; o >= 1; o >>= 1) val = fmaxf(val, __shfl_xor(val, o, kWave));
+    if (lane == leader) atomicMax(&st[lscene].vmax2_bits, __float_as_uint(val));
+    if (pending && my_scene == lscene) pending = false;
+  }
+}
+
+__global__ void k_eos(uint32_t N, const SceneDev* __restrict__ sc, State cur, SceneState* __restrict__ st,
+                      uint32_t n_scenes) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t meta = cur.META[s];
+  uint32_t k = scene_of(meta);
+  const SceneDev& S = sc[k];
+  double c = st[k].c;
+  if (S.adapt) {
+    double vmax = sqrt((double)__uint_as_float(st[k].vmax2_bits));
+    double min_c = 10.0 * vmax;
+    if (c < min_c) c = min_c;
+  }
+  double Bd = S.rho0_d * c * c / S.gamma_d;
+  if (s == S.slot_first) {  // every thread computes the same c; one writes it back
+    st[k].c = c;
+    st[k].B = Bd;
+  }
+  float4 b = cur.B2[s];
+  float P = 0.0f;
+  if (!(meta & kMetaBoundary)) {
+    float rho = cur.A2[s].w;
+    float p = tait_p(rho, S, (float)Bd);
+    P = p / (rho * rho);
+  }
+  cur.B2[s] = make_float4(b.x, b.y, b.z, P);
+  (void)n_scenes;
+}
+
+// ---------------------------------------------------------------------------
+// pressure + viscosity + Akinci boundary forces, gravity, kick and drift
+// (wcsph.rs:181-259, 331-342). Gather form: every fluid particle sums its own
+// pair terms; boundary particles gather their reaction force.
+// ---------------------------------------------------------------------------
+__global__ void k_force(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
+                        const SceneState* __restrict__ st, State cur, const uint32_t* __restrict__ LIST,
+                        const uint32_t* __restrict__ CNT, uint32_t cap, int integrate, int with_boundary) {
+  uint32_t G = gridDim.x;
+  uint32_t blk = xcd_block(blockIdx.x, G);
+  uint32_t s = blk * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t meta = cur.META[s];
+  const SceneDev& S = sc[scene_of(meta)];
+  float4 ai = cur.A2[s];
+  float4 bi = cur.B2[s];
+  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, aoh = S.alpha * S.inv_h;
+  uint32_t n = CNT[s];
+  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
+  cur.IDF[s] = cur.ID[s];
+  if (meta & kMetaBoundary) {
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (with_boundary) {
+      const float nm = -S.mass;
+      for (uint32_t k = 0; k < n; ++k) {
+        uint32_t e = lp[(uint64_t)k * kWave];
+        uint32_t j = e & kSlotMask;
+        float4 aj = cur.A2[j];
+        float ox, oy, oz;
+        entry_offset(e, L, ox, oy, oz);
+        // from the fluid particle's view: r = x_b - x_f
+        float rx = ai.x - (aj.x + ox), ry = ai.y - (aj.y + oy), rz = ai.z - (aj.z + oz);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        if (!(r2 < r2s) || !(r2 > 0.0f)) continue;
+        float r = __builtin_amdgcn_sqrtf(r2);
+        float gf = -cubic_dwdr(r * inv_h, aoh) * __builtin_amdgcn_rcpf(r);
+        float Pj = cur.B2[j].w;
+        float sfac = gf * (ai.w * Pj) * nm;  // gw * (-(rho0 psi_b) P_f) * (-m)
+        fx += rx * sfac;
+        fy += ry * sfac;
+        fz += rz * sfac;
+      }
+    }
+    cur.BF[s] = make_float4(fx, fy, fz, 0.0f);
+    cur.ACC[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float m = S.mass, h = S.h, va = S.visc_alpha, vb = S.visc_beta, eh2 = S.visc_eps_h2;
+  const float c = (float)st[scene_of(meta)].c;
+  const float Pi = bi.w, rhoi = ai.w;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t e = lp[(uint64_t)k * kWave];
+    uint32_t j = e & kSlotMask;
+    float4 aj = cur.A2[j];
+    float ox, oy, oz;
+    entry_offset(e, L, ox, oy, oz);
+    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;  // x_j - x_i
+    float r2 = rx * rx + ry * ry + rz * rz;
+    if (!(r2 < r2s) || !(r2 > 0.0f)) continue;
+    float4 bj = cur.B2[j];
+    float r = __builtin_amdgcn_sqrtf(r2);
+    float gf = -cubic_dwdr(r * inv_h, aoh) * __builtin_amdgcn_rcpf(r);
+    float sfac;
+    if (!is_boundary_w(aj.w)) {
+      float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+      float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
+      float pi = 0.0f;
+      if (vr < 0.0f) {
+        float mu = h * vr * __builtin_amdgcn_rcpf(r2 + eh2);
+        pi = (-va * c * mu + vb * mu * mu) * __builtin_amdgcn_rcpf(0.5f * (rhoi + aj.w));
+      }
+      sfac = gf * (-m * ((Pi + bj.w) + pi));
+    } else {
+      sfac = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
+    }
+    ax += rx * sfac;
+    ay += ry * sfac;
+    az += rz * sfac;
+  }
+  ax += S.g[0];
+  ay += S.g[1];
+  az += S.g[2];
+  cur.ACC[s] = make_float4(ax, ay, az, 0.0f);
+  cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!integrate) return;
+  const double dt = ctl->dt;
+  const float dtf = (float)dt;
+  float vx = bi.x + ax * dtf, vy = bi.y + ay * dtf, vz = bi.z + az * dtf;
+  if (S.dim == 2) vz = 0.0f;
+  double x = cur.X[s] + (double)vx * dt;
+  double y = cur.Y[s] + (double)vy * dt;
+  double z = (S.dim == 2) ? 0.0 : cur.Z[s] + (double)vz * dt;
+  cur.X[s] = x;
+  cur.Y[s] = y;
+  cur.Z[s] = z;
+  uint32_t local = cur.CELL[s] - S.cell_base;
+  uint32_t lz = local % (uint32_t)S.nz;
+  uint32_t t = local / (uint32_t)S.nz;
+  uint32_t ly = t % (uint32_t)S.ny;
+  uint32_t lx = t / (uint32_t)S.ny;
+  double cx = (double)(S.ix0 + (int32_t)lx) * (double)S.cell_f;
+  double cy = (double)(S.iy0 + (int32_t)ly) * (double)S.cell_f;
+  double cz = (double)(S.iz0 + (int32_t)lz) * (double)S.cell_f;
+  cur.A[s] = make_float4((float)(x - cx), (float)(y - cy), (float)(z - cz), m);
+  cur.B[s] = make_float4(vx, vy, vz, 0.0f);
+  float d = cur.DISP[s] + sqrtf(vx * vx + vy * vy + vz * vz) * dtf;
+  cur.DISP[s] = d;
+  unsigned long long over = __ballot(!(d <= S.skin_half));  // also catches NaN; active lanes only
+  if (over != 0ull && (int)(threadIdx.x & (kWave - 1)) == __ffsll((long long)over) - 1) ctlw->need_rebuild = 1u;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics (diagnostics.rs:41-102, wcsph.rs:350-369)
+// ---------------------------------------------------------------------------
+constexpr int kDiagThreads = 256;
+
+__global__ void __launch_bounds__(kDiagThreads) k_diag_partial(uint32_t first, uint32_t count, const SceneDev* __restrict__ sc,
+                                                               uint32_t scene, State cur, DiagAcc* __restrict__ part) {
+  const SceneDev& S = sc[scene];
+  DiagAcc a = {};
+  const double m = (double)S.mass;
+  const double gmag = sqrt((double)S.g[0] * S.g[0] + (double)S.g[1] * S.g[1] + (double)S.g[2] * S.g[2]);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+    uint32_t s = first + k;
+    if (cur.META[s] & kMetaBoundary) continue;
+    float4 v = cur.B2[s];
+    double vx = v.x, vy = v.y, vz = v.z;
+    a.mom[0] += vx * m;
+    a.mom[1] += vy * m;
+    a.mom[2] += vz * m;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    a.ke += 0.5 * m * v2;
+    a.pe += m * gmag * cur.Z[s];
+    double e = fabs((double)cur.A2[s].w - S.rho0_d) / S.rho0_d;
+    a.derr_sum += e;
+    a.derr_max = fmax(a.derr_max, e);
+    a.v2max = fmax(a.v2max, v2);
+    float4 acc = cur.ACC[s];
+    double a2 = (double)acc.x * acc.x + (double)acc.y * acc.y + (double)acc.z * acc.z;
+    a.a2max = fmax(a.a2max, a2);
+    a.nfluid += 1.0;
+  }
+  __shared__ DiagAcc sh[kDiagThreads];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = kDiagThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      DiagAcc& x = sh[threadIdx.x];
+      const DiagAcc& y = sh[threadIdx.x + o];
+      for (int q = 0; q < 3; ++q) x.mom[q] += y.mom[q];
+      x.ke += y.ke;
+      x.pe += y.pe;
+      x.derr_sum += y.derr_sum;
+      x.derr_max = fmax(x.derr_max, y.derr_max);
+      x.v2max = fmax(x.v2max, y.v2max);
+      x.a2max = fmax(x.a2max, y.a2max);
+      x.nfluid += y.nfluid;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_diag_final(const DiagAcc* __restrict__ part, uint32_t nb, DiagAcc* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DiagAcc a = {};
+  for (uint32_t b = 0; b < nb; ++b) {  // fixed order: deterministic
+    const DiagAcc& y = part[b];
+    for (int q = 0; q < 3; ++q) a.mom[q] += y.mom[q];
+    a.ke += y.ke;
+    a.pe += y.pe;
+    a.derr_sum += y.derr_sum;
+    a.derr_max = fmax(a.derr_max, y.derr_max);
+    a.v2max = fmax(a.v2max, y.v2max);
+    a.a2max = fmax(a.a2max, y.a2max);
+    a.nfluid += y.nfluid;
+  }
+  *out = a;
+}
+
+// ---------------------------------------------------------------------------
+// getters: sorted slots -> original order
+// ---------------------------------------------------------------------------
+// what: 0 x(f64), 1 v(B2), 2 acc(ACC via IDF), 3 rho (A2.w or rho0), 4 p, 5 boundary force (BF via IDF),
+//       6 psi (-A.w / rho0 for boundary), 7 n_out, 8 kind
+__global__ void k_unsort(uint32_t N, const SceneDev* __restrict__ sc, const SceneState* __restrict__ st, State cur,
+                         int what, double* __restrict__ out) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t meta = cur.META[s];
+  const SceneDev& S = sc[scene_of(meta)];
+  const bool bnd = (meta & kMetaBoundary) != 0u;
+  uint32_t id = (what == 2 || what == 5) ? cur.IDF[s] : cur.ID[s];
+  double* o3 = out + 3ull * id;
+  switch (what) {
+    case 0:
+      o3[0] = cur.X[s];
+      o3[1] = cur.Y[s];
+      o3[2] = cur.Z[s];
+      break;
+    case 1: {
+      float4 v = cur.B2[s];
+      o3[0] = v.x;
+      o3[1] = v.y;
+      o3[2] = v.z;
+    } break;
+    case 2: {
+      float4 a = cur.ACC[s];
+      o3[0] = a.x;
+      o3[1] = a.y;
+      o3[2] = a.z;
+    } break;
+    case 3:
+      out[id] = bnd ? S.rho0_d : (double)cur.A2[s].w;
+      break;
+    case 4:
+      out[id] = bnd ? 0.0 : (double)tait_p(cur.A2[s].w, S, (float)st[scene_of(meta)].B);
+      break;
+    case 5: {
+      float4 f = cur.BF[s];
+      o3[0] = f.x;
+      o3[1] = f.y;
+      o3[2] = f.z;
+    } break;
+    case 6:
+      out[id] = bnd ? -(double)cur.A[s].w / S.rho0_d : 0.0;
+      break;
+    case 7:
+      if (meta & kMetaOriented) {
+        o3[0] = cur.NX[s];
+        o3[1] = cur.NY[s];
+        o3[2] = cur.NZ[s];
+      } else {
+        o3[0] = o3[1] = o3[2] = 0.0;
+      }
+      break;
+    case 8:
+      out[id] = bnd ? 1.0 : 0.0;
+      break;
+    default:
+      break;
+  }
+}
+
+// Upload: particles in original order land in slot = original index.
+__global__ void k_stage(uint32_t N, const SceneDev* __restrict__ sc, State cur, const double* __restrict__ x,
+                        const double* __restrict__ v, const uint8_t* __restrict__ kind, const double* __restrict__ psi,
+                        const double* __restrict__ nrm, const uint32_t* __restrict__ scene_of_id) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  uint32_t k = scene_of_id[i];
+  const SceneDev& S = sc[k];
+  const bool bnd = kind[i] != 0;
+  double nx = nrm[3ull * i], ny = nrm[3ull * i + 1], nz = nrm[3ull * i + 2];
+  const bool oriented = bnd && (nx * nx + ny * ny + nz * nz) > 0.0;
+  uint32_t meta = (k << kMetaSceneShift) | (bnd ? kMetaBoundary : 0u) | (oriented ? kMetaOriented : 0u);
+  cur.X[i] = x[3ull * i];
+  cur.Y[i] = x[3ull * i + 1];
+  cur.Z[i] = x[3ull * i + 2];
+  float vx = bnd ? 0.0f : (float)v[3ull * i], vy = bnd ? 0.0f : (float)v[3ull * i + 1],
+        vz = bnd ? 0.0f : (float)v[3ull * i + 2];
+  cur.B[i] = make_float4(vx, vy, vz, 0.0f);
+  cur.B2[i] = make_float4(vx, vy, vz, 0.0f);
+  float w = bnd ? -(float)(S.rho0_d * psi[i]) : S.mass;
+  cur.A[i] = make_float4(0.0f, 0.0f, 0.0f, w);
+  cur.A2[i] = make_float4(0.0f, 0.0f, 0.0f, bnd ? w : S.rho0);
+  cur.ID[i] = i;
+  cur.IDF[i] = i;
+  cur.META[i] = meta;
+  cur.DISP[i] = 0.0f;
+  cur.NX[i] = nx;
+  cur.NY[i] = ny;
+  cur.NZ[i] = nz;
+  cur.ACC[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cur.BF[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// reference grid probe (cell = 2h): key bounds, occupied-cell compaction,
+// candidate lists of one particle, filtered list extraction
+// ---------------------------------------------------------------------------
+__global__ void k_key_bounds(uint32_t N, const SceneDev* __restrict__ sc, State cur, int32_t* __restrict__ bounds) {
+  // bounds per scene: 6 ints (min x, y, z as -key via atomicMax of -k, max x, y, z)
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t meta = cur.META[s];
+  uint32_t k = scene_of(meta);
+  const SceneDev& S = sc[k];
+  int32_t kx = floor_i32(cur.X[s] * S.inv_cell);
+  int32_t ky = floor_i32(cur.Y[s] * S.inv_cell);
+  int32_t kz = (S.dim == 2) ? 0 : floor_i32(cur.Z[s] * S.inv_cell);
+  int32_t* b = bounds + 6 * k;
+  atomicMin(&b[0], kx);
+  atomicMin(&b[1], ky);
+  atomicMin(&b[2], kz);
+  atomicMax(&b[3], kx);
+  atomicMax(&b[4], ky);
+  atomicMax(&b[5], kz);
+}
+
+// For the probe: perm holds the sorted slots; write the reference's order
+// (original indices) and the key of every particle (original index).
+__global__ void k_probe_order(uint32_t N, const SceneDev* __restrict__ sc, State cur,
+                              const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cell_start,
+                              const uint32_t* __restrict__ perm, uint64_t* __restrict__ order,
+                              int32_t* __restrict__ keys, uint32_t* __restrict__ slot_of_rank) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t c = cell_of[s];
+  uint32_t my_id = cur.ID[s];
+  uint32_t dst = ranked_dst(s, c, my_id, cell_start, perm, cur.ID);
+  order[dst] = my_id;
+  slot_of_rank[dst] = s;
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  uint32_t local = c - S.cell_base;
+  uint32_t lz = local % (uint32_t)S.nz;
+  uint32_t t = local / (uint32_t)S.nz;
+  uint32_t ly = t % (uint32_t)S.ny;
+  uint32_t lx = t / (uint32_t)S.ny;
+  keys[3ull * my_id] = S.ix0 + (int32_t)lx;
+  keys[3ull * my_id + 1] = S.iy0 + (int32_t)ly;
+  keys[3ull * my_id + 2] = S.iz0 + (int32_t)lz;
+}
+
+__global__ void k_flag_nonempty(uint32_t C, const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ flag) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) flag[c] = (cell_start[c + 1] > cell_start[c]) ? 1u : 0u;
+}
+
+__global__ void k_compact_cells(uint32_t C, const SceneDev* __restrict__ sc, uint32_t n_scenes,
+                                const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ flag_pos,
+                                int32_t* __restrict__ occupied, uint64_t* __restrict__ ranges) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  uint32_t b = cell_start[c], e = cell_start[c + 1];
+  if (e <= b) return;
+  uint32_t k = 0;
+  while (k + 1 < n_scenes && c >= sc[k + 1].cell_base) ++k;
+  const SceneDev& S = sc[k];
+  uint32_t local = c - S.cell_base;
+  uint32_t lz = local % (uint32_t)S.nz;
+  uint32_t t = local / (uint32_t)S.nz;
+  uint32_t ly = t % (uint32_t)S.ny;
+  uint32_t lx = t / (uint32_t)S.ny;
+  uint32_t o = flag_pos[c];
+  occupied[3ull * o] = S.ix0 + (int32_t)lx;
+  occupied[3ull * o + 1] = S.iy0 + (int32_t)ly;
+  occupied[3ull * o + 2] = S.iz0 + (int32_t)lz;
+  ranges[2ull * o] = b;
+  ranges[2ull * o + 1] = e;
+}
+
+// Candidate list of one particle (grid.rs:122-140): one thread writes the
+// cells in dz -> dy -> dx order, ascending index within a cell.
+__global__ void k_probe_candidates(const SceneDev* __restrict__ sc, uint32_t scene, const int32_t* __restrict__ key,
+                                   const uint32_t* __restrict__ cell_start, const uint64_t* __restrict__ order,
+                                   uint64_t* __restrict__ out, uint64_t cap, uint64_t* __restrict__ out_len) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const SceneDev& S = sc[scene];
+  int32_t lx = key[0] - S.ix0, ly = key[1] - S.iy0, lz = key[2] - S.iz0;
+  uint64_t n = 0;
+  const int zlo = (S.dim == 2) ? 0 : -1, zhi = (S.dim == 2) ? 0 : 1;
+  for (int dz = zlo; dz <= zhi; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int32_t x = lx + dx, y = ly + dy, z = lz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= S.nx || y >= S.ny || z >= S.nz) continue;
+        uint32_t cc = S.cell_base + ((uint32_t)x * (uint32_t)S.ny + (uint32_t)y) * (uint32_t)S.nz + (uint32_t)z;
+        for (uint32_t p = cell_start[cc]; p < cell_start[cc + 1]; ++p) {
+          if (n < cap) out[n] = order[p];
+          ++n;
+        }
+      }
+  *out_len = n;
+}
+
+// The device list of the particle in `slot`, filtered to r < 2h, as original
+// indices (unsorted; the host orders them for presentation).
+__global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, State cur,
+                               const uint32_t* __restrict__ LIST, const uint32_t* __restrict__ CNT, uint32_t cap,
+                               uint64_t* __restrict__ out, uint32_t* __restrict__ out_len) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const SceneDev& S = sc[scene_of(cur.META[slot])];
+  float4 ai = cur.A[slot];
+  uint32_t n = CNT[slot], m = 0;
+  const uint32_t* lp = LIST + ((uint64_t)(slot >> 6) * cap) * kWave + (slot & 63u);
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t e = lp[(uint64_t)k * kWave];
+    uint32_t j = e & kSlotMask;
+    float4 aj = cur.A[j];
+    float ox, oy, oz;
+    entry_offset(e, S.cell_f, ox, oy, oz);
+    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
+    float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 < S.r2_sup) out[m++] = cur.ID[j];
+  }
+  *out_len = m;
+}
+
+// slot of each original index
+__global__ void k_slot_of_id(uint32_t N, State cur, uint32_t* __restrict__ slot_of_id) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) slot_of_id[cur.ID[s]] = s;
+}
+
+// mean list length
+__global__ void k_list_stats(uint32_t N, const uint32_t* __restrict__ CNT, Ctl* ctl) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t v = (s < N) ? CNT[s] : 0u;
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&ctl->list_entries, (unsigned long long)v);
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+#define PEF_LAUNCH(kernel, grid, block, stream, ...) \
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n) {
+  if (n == 0) return;
+  PEF_LAUNCH(k_fill_u32, blocks_for(n, 256), 256, st, p, v, n);
+}
+
+void launch_decide(hipStream_t st, Ctl* ctl, SceneState* ss, uint32_t n_scenes) {
+  PEF_LAUNCH(k_decide, 1, 64, st, ctl, ss, n_scenes);
+}
+
+void launch_grid_build(hipStream_t st, const GridArgs& g, int always) {
+  const uint32_t N = g.N, C = g.C;
+  const uint32_t bs = 256;
+  PEF_LAUNCH(k_cell_count, blocks_for(N, bs), bs, st, N, g.ctl, g.ctl, g.sc, g.meta, g.X, g.Y, g.Z, g.cell_of,
+             g.rank_of, g.counts, always);
+  uint32_t nb = blocks_for(C, kScanTile);
+  PEF_LAUNCH(k_scan_local, nb, kScanThreads, st, g.ctl, always, g.counts, g.cell_start, g.block_sums, C);
+  PEF_LAUNCH(k_scan_sums, 1, kScanThreads, st, g.ctl, always, g.block_sums, nb);
+  PEF_LAUNCH(k_scan_add, blocks_for(C + 1, bs), bs, st, g.ctl, always, g.cell_start, g.block_sums, C, nb);
+  PEF_LAUNCH(k_scatter, blocks_for(N, bs), bs, st, N, g.ctl, always, g.cell_of, g.rank_of, g.cell_start, g.perm);
+}
+
+void launch_finalize(hipStream_t st, uint32_t N, const Ctl* ctl, const SceneDev* sc, const uint32_t* cell_of,
+                     const uint32_t* cell_start, const uint32_t* perm, const State& cur, const State& alt,
+                     uint32_t* counts, uint32_t C) {
+  const uint32_t bs = 256;
+  PEF_LAUNCH(k_finalize, blocks_for(N, bs), bs, st, N, ctl, sc, cell_of, cell_start, perm, cur, alt);
+  uint64_t nmax = N > C ? N : C;
+  PEF_LAUNCH(k_copyback, blocks_for(nmax, bs), bs, st, N, ctl, alt, cur, counts, C);
+}
+
+void launch_nlist(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc,
+                  const uint32_t* cell_start, const State& cur, uint32_t* LIST, uint32_t* CNT, uint32_t cap,
+                  int with_boundary, int always, uint32_t bs) {
+  PEF_LAUNCH(k_nlist, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, cell_start, cur, LIST, CNT, cap, with_boundary,
+             always);
+}
+
+void launch_density(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                    const uint32_t* CNT, uint32_t cap, uint32_t bs) {
+  PEF_LAUNCH(k_density, blocks_for(N, bs), bs, st, N, sc, cur, LIST, CNT, cap);
+}
+
+void launch_xsph(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                 const uint32_t* CNT, uint32_t cap, uint32_t bs) {
+  PEF_LAUNCH(k_xsph, blocks_for(N, bs), bs, st, N, sc, cur, LIST, CNT, cap);
+}
+
+void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, SceneState* ss,
+                     uint32_t n_scenes, uint32_t* vmax_bits_fill_base) {
+  const uint32_t bs = 256;
+  (void)vmax_bits_fill_base;
+  PEF_LAUNCH(k_vmax_zero, blocks_for(n_scenes, 64), 64, st, ss, n_scenes);
+  PEF_LAUNCH(k_vmax, blocks_for(N, bs), bs, st, N, cur.B2, cur.META, ss);
+  PEF_LAUNCH(k_eos, blocks_for(N, bs), bs, st, N, sc, cur, ss, n_scenes);
+}
+
+void launch_force(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc, const SceneState* ss,
+                  const State& cur, const uint32_t* LIST, const uint32_t* CNT, uint32_t cap, int integrate,
+                  int with_boundary, uint32_t bs) {
+  PEF_LAUNCH(k_force, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, ss, cur, LIST, CNT, cap, integrate,
+             with_boundary);
+}
+
+void launch_diag(hipStream_t st, uint32_t first, uint32_t count, const SceneDev* sc, uint32_t scene, const State& cur,
+                 DiagAcc* part, uint32_t nb, DiagAcc* out) {
+  PEF_LAUNCH(k_diag_partial, nb, kDiagThreads, st, first, count, sc, scene, cur, part);
+  PEF_LAUNCH(k_diag_final, 1, 64, st, part, nb, out);
+}
+
+void launch_unsort(hipStream_t st, uint32_t N, const SceneDev* sc, const SceneState* ss, const State& cur, int what,
+                   double* out) {
+  PEF_LAUNCH(k_unsort, blocks_for(N, 256), 256, st, N, sc, ss, cur, what, out);
+}
+
+void launch_stage(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const double* x,
+                  const double* v, const uint8_t* kind, const double* psi, const double* nrm,
+                  const uint32_t* scene_of_id) {
+  PEF_LAUNCH(k_stage, blocks_for(N, 256), 256, st, N, sc, cur, x, v, kind, psi, nrm, scene_of_id);
+}
+
+void launch_key_bounds(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, int32_t* bounds) {
+  PEF_LAUNCH(k_key_bounds, blocks_for(N, 256), 256, st, N, sc, cur, bounds);
+}
+
+void launch_probe_order(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* cell_of,
+                        const uint32_t* cell_start, const uint32_t* perm, uint64_t* order, int32_t* keys,
+                        uint32_t* slot_of_rank) {
+  PEF_LAUNCH(k_probe_order, blocks_for(N, 256), 256, st, N, sc, cur, cell_of, cell_start, perm, order, keys,
+             slot_of_rank);
+}
+
+void launch_compact_cells(hipStream_t st, uint32_t C, const SceneDev* sc, uint32_t n_scenes,
+                          const uint32_t* cell_start, uint32_t* flag, uint32_t* flag_pos, uint32_t* block_sums,
+                          int32_t* occupied, uint64_t* ranges, const Ctl* ctl) {
+  PEF_LAUNCH(k_flag_nonempty, blocks_for(C, 256), 256, st, C, cell_start, flag);
+  uint32_t nb = blocks_for(C, kScanTile);
+  PEF_LAUNCH(k_scan_local, nb, kScanThreads, st, ctl, 1, flag, flag_pos, block_sums, C);
+  PEF_LAUNCH(k_scan_sums, 1, kScanThreads, st, ctl, 1, block_sums, nb);
+  PEF_LAUNCH(k_scan_add, blocks_for(C + 1, 256), 256, st, ctl, 1, flag_pos, block_sums, C, nb);
+  if (occupied)
+    PEF_LAUNCH(k_compact_cells, blocks_for(C, 256), 256, st, C, sc, n_scenes, cell_start, flag_pos, occupied, ranges);
+}
+
+void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene, const int32_t* key,
+                             const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
+                             uint64_t* out_len) {
+  PEF_LAUNCH(k_probe_candidates, 1, 64, st, sc, scene, key, cell_start, order, out, cap, out_len);
+}
+
+void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                         const uint32_t* CNT, uint32_t cap, uint64_t* out, uint32_t* out_len) {
+  PEF_LAUNCH(k_extract_list, 1, 64, st, slot, sc, cur, LIST, CNT, cap, out, out_len);
+}
+
+void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id) {
+  PEF_LAUNCH(k_slot_of_id, blocks_for(N, 256), 256, st, N, cur, slot_of_id);
+}
+
+void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl) {
+  PEF_LAUNCH(k_list_stats, blocks_for(N, 256), 256, st, N, CNT, ctl);
+}
+
+}  // namespace pef

@@ -1,0 +1,76 @@
+// Host-side launchers of the WCSPH kernels (defined in pef_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pef_common.h"
+
+namespace pef {
+
+// Device particle state, one entry per slot (layout in pef_common.h).
+struct State {
+  double *X, *Y, *Z;
+  float4 *A, *A2, *B, *B2;
+  uint32_t *ID, *IDF, *META, *CELL;
+  float* DISP;
+  double *NX, *NY, *NZ;
+  float4 *ACC, *BF;
+};
+
+struct GridArgs {
+  uint32_t N, C;
+  Ctl* ctl;
+  const SceneDev* sc;
+  const uint32_t* meta;
+  const double *X, *Y, *Z;
+  uint32_t *cell_of, *rank_of, *counts, *cell_start, *block_sums, *perm;
+};
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
+void launch_decide(hipStream_t st, Ctl* ctl, SceneState* ss, uint32_t n_scenes);
+void launch_grid_build(hipStream_t st, const GridArgs& g, int always);
+void launch_finalize(hipStream_t st, uint32_t N, const Ctl* ctl, const SceneDev* sc, const uint32_t* cell_of,
+                     const uint32_t* cell_start, const uint32_t* perm, const State& cur, const State& alt,
+                     uint32_t* counts, uint32_t C);
+void launch_nlist(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc,
+                  const uint32_t* cell_start, const State& cur, uint32_t* LIST, uint32_t* CNT, uint32_t cap,
+                  int with_boundary, int always, uint32_t bs);
+void launch_density(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                    const uint32_t* CNT, uint32_t cap, uint32_t bs);
+void launch_xsph(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                 const uint32_t* CNT, uint32_t cap, uint32_t bs);
+void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, SceneState* ss,
+                     uint32_t n_scenes, uint32_t* unused);
+void launch_force(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc, const SceneState* ss,
+                  const State& cur, const uint32_t* LIST, const uint32_t* CNT, uint32_t cap, int integrate,
+                  int with_boundary, uint32_t bs);
+void launch_diag(hipStream_t st, uint32_t first, uint32_t count, const SceneDev* sc, uint32_t scene, const State& cur,
+                 DiagAcc* part, uint32_t nb, DiagAcc* out);
+void launch_unsort(hipStream_t st, uint32_t N, const SceneDev* sc, const SceneState* ss, const State& cur, int what,
+                   double* out);
+void launch_stage(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const double* x,
+                  const double* v, const uint8_t* kind, const double* psi, const double* nrm,
+                  const uint32_t* scene_of_id);
+void launch_key_bounds(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, int32_t* bounds);
+void launch_probe_order(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* cell_of,
+                        const uint32_t* cell_start, const uint32_t* perm, uint64_t* order, int32_t* keys,
+                        uint32_t* slot_of_rank);
+void launch_compact_cells(hipStream_t st, uint32_t C, const SceneDev* sc, uint32_t n_scenes,
+                          const uint32_t* cell_start, uint32_t* flag, uint32_t* flag_pos, uint32_t* block_sums,
+                          int32_t* occupied, uint64_t* ranges, const Ctl* ctl);
+void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene, const int32_t* key,
+                             const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
+                             uint64_t* out_len);
+void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint32_t* LIST,
+                         const uint32_t* CNT, uint32_t cap, uint64_t* out, uint32_t* out_len);
+void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id);
+void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl);
+
+// pef_setup.hip
+int akinci_psi_gpu(hipStream_t st, int dim, double h, uint64_t nb, const double* bx_host, double* psi_host);
+
+// scan tile: number of cells one scan block covers
+constexpr uint32_t kScanTileHost = 8192;
+
+}  // namespace pef

@@ -1,0 +1,1088 @@
+// PefWorld: host orchestration of the device WCSPH step and the C ABI (pef.h).
+//
+// One world owns one HIP stream, the device particle state of all its scenes,
+// the neighbour lists and the control block. A step is the fixed kernel
+// sequence below; multi-step calls replay it from a captured HIP graph.
+//
+//   step body (steady state; wcsph.rs:326-347 with the first rebuild_grid and
+//   compute_density of each step reused from the previous step's second ones,
+//   which see identical positions):
+//     k_vmax_zero, k_vmax, k_eos          compute_pressure
+//     k_force (integrate)                 compute_forces + kick + drift
+//     k_decide                            list rebuild needed? (skin criterion)
+//     k_cell_count .. k_copyback          rebuild_grid (when decided)
+//     k_nlist                             neighbour lists (when decided)
+//     k_density                           compute_density
+//     k_xsph                              apply_xsph
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pef.h"
+#include "pef_common.h"
+#include "pef_launch.h"
+
+namespace pef {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+class World {
+ public:
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t N = 0, S = 0, C = 0;
+  uint32_t cap = 384;
+  double skin = 0.08;
+  uint32_t margin = 4;
+  int with_boundary = 1;
+  int use_graph = 1;
+  uint32_t bs = 256;
+  std::vector<PefParams> params;
+  std::vector<SceneDev> scenes;
+  std::vector<uint64_t> scene_first, scene_count;
+  SceneDev* sc_d = nullptr;
+  SceneState* st_d = nullptr;
+  Ctl* ctl_d = nullptr;
+  State cur{}, alt{};
+  uint32_t *cell_of = nullptr, *rank_of = nullptr, *counts = nullptr, *cell_start = nullptr, *block_sums = nullptr,
+           *perm = nullptr;
+  uint32_t *LIST = nullptr, *CNT = nullptr;
+  double* stage = nullptr;  // 3N doubles for getters
+  std::vector<void*> allocs;
+  double t = 0.0;
+  bool density_valid = false;
+  // static per-particle inputs in original order (change only through set_state)
+  std::vector<double> h_psi, h_nout;
+  std::vector<uint8_t> h_kind;
+  hipGraphExec_t graph = nullptr;
+  bool profiling = false;
+  std::vector<std::string> prof_names;
+  std::vector<double> prof_ms;
+  // probe (reference grid) cache
+  bool probe_valid = false;
+  std::vector<SceneDev> probe_sc_h;
+  SceneDev* probe_sc = nullptr;
+  uint32_t probe_C = 0;
+  uint32_t *probe_counts = nullptr, *probe_start = nullptr, *probe_sums = nullptr, *probe_flag = nullptr,
+           *probe_flagpos = nullptr, *probe_slot_of_rank = nullptr;
+  uint64_t* probe_order = nullptr;
+  int32_t* probe_keys = nullptr;
+
+  ~World() { release(); }
+
+  template <class T>
+  hipError_t dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e == hipSuccess) allocs.push_back((void*)*p);
+    return e;
+  }
+  void release() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    release_probe();
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+  void release_probe() {
+    void* ps[] = {probe_sc, probe_counts, probe_start, probe_sums, probe_flag, probe_flagpos, probe_slot_of_rank,
+                  probe_order, probe_keys};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    probe_sc = nullptr;
+    probe_counts = probe_start = probe_sums = probe_flag = probe_flagpos = probe_slot_of_rank = nullptr;
+    probe_order = nullptr;
+    probe_keys = nullptr;
+    probe_valid = false;
+  }
+};
+
+}  // namespace pef
+
+using pef::World;
+
+struct PefWorld {
+  World w;
+};
+
+namespace {
+
+int32_t hip_status(hipError_t e) { return e == hipSuccess ? PEF_OK : PEF_ERR_PANIC; }
+
+#define HIPTRY(x)                                     \
+  do {                                                \
+    hipError_t _e = (x);                              \
+    if (_e != hipSuccess) {                           \
+      std::fprintf(stderr, "pef: %s failed: %s\n", #x, \
+                   hipGetErrorString(_e));            \
+      return PEF_ERR_PANIC;                           \
+    }                                                 \
+  } while (0)
+
+bool finite3(const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int32_t check_params(const PefParams& p) {
+  if (p.dim != 2 && p.dim != 3) return PEF_ERR_ARG;
+  if (p.kernel != PEF_KERNEL_CUBIC_SPLINE) return PEF_ERR_UNSUPPORTED;
+  if (p.surface_tension_sigma != 0.0) return PEF_ERR_UNSUPPORTED;
+  const double pos[] = {p.rest_density, p.mass, p.h, p.gamma};
+  for (double v : pos)
+    if (!(std::isfinite(v) && v > 0.0)) return PEF_ERR_ARG;
+  const double fin[] = {p.sound_speed, p.visc_alpha, p.visc_beta, p.visc_eps, p.xsph_epsilon};
+  for (double v : fin)
+    if (!std::isfinite(v)) return PEF_ERR_ARG;
+  if (!finite3(p.gravity)) return PEF_ERR_ARG;
+  return PEF_OK;
+}
+
+// Per-scene device constants from params and the grid placement.
+void fill_scene_physics(pef::SceneDev& S, const PefParams& p, double skin) {
+  S.dim = p.dim;
+  S.adapt = p.adapt_sound_speed ? 1 : 0;
+  double g = p.gamma;
+  S.gamma_int = (g == std::floor(g) && g >= 1.0 && g <= 16.0) ? (int32_t)g : 0;
+  S.h = (float)p.h;
+  S.inv_h = (float)(1.0 / p.h);
+  S.mass = (float)p.mass;
+  S.rho0 = (float)p.rest_density;
+  S.inv_rho0 = (float)(1.0 / p.rest_density);
+  const double pi = 3.14159265358979323846;
+  S.alpha = (float)((p.dim == 2) ? 10.0 / (7.0 * pi * p.h * p.h) : 1.0 / (pi * p.h * p.h * p.h));
+  double sup = 2.0 * p.h;
+  double rl = sup * (1.0 + skin);
+  S.r2_list = (float)(rl * rl);
+  S.r2_sup = (float)(sup * sup);
+  S.visc_alpha = (float)p.visc_alpha;
+  S.visc_beta = (float)p.visc_beta;
+  S.visc_eps_h2 = (float)(p.visc_eps * p.h * p.h);
+  S.xsph_eps = (float)p.xsph_epsilon;
+  S.gamma = (float)p.gamma;
+  for (int a = 0; a < 3; ++a) S.g[a] = (float)p.gravity[a];
+  S.skin_half = (float)(0.5 * skin * sup);
+  S.h_d = p.h;
+  S.rho0_d = p.rest_density;
+  S.gamma_d = p.gamma;
+}
+
+int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefParticles* parts, const PefOptions* opt,
+                     PefWorld** out) {
+  if (!params || !parts || !out) return PEF_ERR_NULL;
+  *out = nullptr;
+  if (n_scenes == 0) return PEF_ERR_ARG;
+  uint64_t Ntot = 0;
+  for (uint32_t k = 0; k < n_scenes; ++k) {
+    int32_t rc = check_params(params[k]);
+    if (rc != PEF_OK) return rc;
+    const PefParticles& P = parts[k];
+    if (P.n > 0 && (!P.x || !P.kind)) return PEF_ERR_NULL;
+    for (uint64_t i = 0; i < P.n; ++i) {
+      if (!finite3(P.x + 3 * i)) return PEF_ERR_ARG;
+      if (P.kind[i] > 1) return PEF_ERR_ARG;
+      if (P.v && !finite3(P.v + 3 * i)) return PEF_ERR_ARG;
+      if (P.psi && !std::isfinite(P.psi[i])) return PEF_ERR_ARG;
+      if (P.n_out && !finite3(P.n_out + 3 * i)) return PEF_ERR_ARG;
+    }
+    Ntot += P.n;
+  }
+  if (Ntot == 0) return PEF_ERR_ARG;
+  if (Ntot >= pef::kMaxSlots) return PEF_ERR_BOUNDS;
+  if (n_scenes >= (1u << 24)) return PEF_ERR_BOUNDS;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PEF_ERR_DEVICE;
+
+  PefWorld* pw = new (std::nothrow) PefWorld();
+  if (!pw) return PEF_ERR_PANIC;
+  World& w = pw->w;
+  auto fail = [&](int32_t rc) {
+    delete pw;
+    return rc;
+  };
+  (void)hipGetDevice(&w.device);
+  if (opt) {
+    if (opt->neighbour_capacity) w.cap = opt->neighbour_capacity;
+    if (opt->skin > 0.0) w.skin = opt->skin;
+    if (opt->skin < 0.0) w.skin = 0.0;
+    if (opt->grid_margin_cells) w.margin = opt->grid_margin_cells;
+    if (opt->boundary_force < 0) w.with_boundary = 0;
+    if (opt->use_graph < 0) w.use_graph = 0;
+    if (opt->block_size) w.bs = opt->block_size;
+  }
+  if (w.cap < 8 || w.cap > 4096 || !(w.skin >= 0.0 && w.skin <= 1.0) || w.bs % 64 != 0 || w.bs > 1024 ||
+      w.margin > 1024)
+    return fail(PEF_ERR_ARG);
+  w.S = n_scenes;
+  w.N = (uint32_t)Ntot;
+  w.params.assign(params, params + n_scenes);
+  w.scenes.resize(n_scenes);
+  w.scene_first.resize(n_scenes);
+  w.scene_count.resize(n_scenes);
+  // grid placement per scene
+  uint64_t cells = 0, first = 0;
+  for (uint32_t k = 0; k < n_scenes; ++k) {
+    const PefParams& p = params[k];
+    const PefParticles& P = parts[k];
+    pef::SceneDev& S = w.scenes[k];
+    std::memset(&S, 0, sizeof(S));
+    fill_scene_physics(S, p, w.skin);
+    S.cell = 2.0 * p.h * (1.0 + w.skin);
+    S.inv_cell = 1.0 / S.cell;
+    S.cell_f = (float)S.cell;
+    int64_t mn[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    bool oriented = false;
+    for (uint64_t i = 0; i < P.n; ++i) {
+      for (int a = 0; a < 3; ++a) {
+        int64_t key = (a == 2 && p.dim == 2) ? 0 : pef::floor_i32(P.x[3 * i + a] * S.inv_cell);
+        mn[a] = std::min(mn[a], key);
+        mx[a] = std::max(mx[a], key);
+      }
+      if (P.n_out && P.kind[i] == 1) {
+        const double* nn = P.n_out + 3 * i;
+        if (nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2] > 0.0) oriented = true;
+      }
+    }
+    if (P.n == 0) mn[0] = mn[1] = mn[2] = mx[0] = mx[1] = mx[2] = 0;
+    int64_t m = (int64_t)w.margin;
+    int64_t lo[3], dims[3];
+    for (int a = 0; a < 3; ++a) {
+      bool flat = (a == 2 && p.dim == 2);
+      lo[a] = flat ? 0 : mn[a] - m;
+      dims[a] = flat ? 1 : (mx[a] - mn[a] + 1 + 2 * m);
+      if (lo[a] < INT32_MIN + 2 || lo[a] + dims[a] > INT32_MAX - 2) return fail(PEF_ERR_BOUNDS);
+    }
+    uint64_t sc_cells = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2];
+    if (sc_cells > (1ull << 30)) return fail(PEF_ERR_BOUNDS);
+    S.ix0 = (int32_t)lo[0];
+    S.iy0 = (int32_t)lo[1];
+    S.iz0 = (int32_t)lo[2];
+    S.nx = (int32_t)dims[0];
+    S.ny = (int32_t)dims[1];
+    S.nz = (int32_t)dims[2];
+    S.cell_base = (uint32_t)cells;
+    S.slot_first = (uint32_t)first;
+    S.slot_count = (uint32_t)P.n;
+    S.id_first = (uint32_t)first;
+    S.oriented = oriented ? 1 : 0;
+    w.scene_first[k] = first;
+    w.scene_count[k] = P.n;
+    cells += sc_cells;
+    first += P.n;
+    if (cells > (1ull << 31)) return fail(PEF_ERR_BOUNDS);
+  }
+  w.C = (uint32_t)cells;
+  if ((uint64_t)(w.C + pef::kScanTileHost - 1) / pef::kScanTileHost > pef::kScanTileHost) return fail(PEF_ERR_BOUNDS);
+  if (hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking) != hipSuccess) return fail(PEF_ERR_DEVICE);
+  const uint32_t N = w.N;
+  const uint64_t chunks = (N + 63) / 64;
+  hipError_t e = hipSuccess;
+#define ALLOC(ptr, cnt) \
+  if (e == hipSuccess) e = w.dalloc(&(ptr), (cnt))
+  ALLOC(w.sc_d, w.S);
+  ALLOC(w.st_d, w.S);
+  ALLOC(w.ctl_d, 1);
+  pef::State* sts[2] = {&w.cur, &w.alt};
+  for (int q = 0; q < 2; ++q) {
+    pef::State& s = *sts[q];
+    ALLOC(s.X, N);
+    ALLOC(s.Y, N);
+    ALLOC(s.Z, N);
+    ALLOC(s.A, N);
+    ALLOC(s.A2, N);
+    ALLOC(s.B, N);
+    ALLOC(s.B2, N);
+    ALLOC(s.ID, N);
+    ALLOC(s.META, N);
+    ALLOC(s.CELL, N);
+    ALLOC(s.DISP, N);
+    ALLOC(s.NX, N);
+    ALLOC(s.NY, N);
+    ALLOC(s.NZ, N);
+  }
+  ALLOC(w.cur.IDF, N);
+  ALLOC(w.cur.ACC, N);
+  ALLOC(w.cur.BF, N);
+  w.alt.IDF = w.cur.IDF;
+  w.alt.ACC = w.cur.ACC;
+  w.alt.BF = w.cur.BF;
+  ALLOC(w.cell_of, N);
+  ALLOC(w.rank_of, N);
+  ALLOC(w.perm, N);
+  ALLOC(w.counts, (size_t)w.C);
+  ALLOC(w.cell_start, (size_t)w.C + 1);
+  ALLOC(w.block_sums, (size_t)(w.C / pef::kScanTileHost) + 2);
+  ALLOC(w.CNT, N);
+  ALLOC(w.LIST, (size_t)chunks * 64 * w.cap);
+  ALLOC(w.stage, 3ull * N);
+#undef ALLOC
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "pef: device allocation failed: %s\n", hipGetErrorString(e));
+    return fail(PEF_ERR_PANIC);
+  }
+  // upload
+  std::vector<double> hx(3ull * N), hv(3ull * N, 0.0), hpsi(N, 0.0), hn(3ull * N, 0.0);
+  std::vector<uint8_t> hk(N);
+  std::vector<uint32_t> hscene(N);
+  for (uint32_t k = 0; k < n_scenes; ++k) {
+    const PefParticles& P = parts[k];
+    uint64_t o = w.scene_first[k];
+    std::memcpy(&hx[3 * o], P.x, 3 * P.n * sizeof(double));
+    if (P.v) std::memcpy(&hv[3 * o], P.v, 3 * P.n * sizeof(double));
+    if (P.psi) std::memcpy(&hpsi[o], P.psi, P.n * sizeof(double));
+    if (P.n_out) std::memcpy(&hn[3 * o], P.n_out, 3 * P.n * sizeof(double));
+    std::memcpy(&hk[o], P.kind, P.n);
+    for (uint64_t i = 0; i < P.n; ++i) hscene[o + i] = k;
+  }
+  std::vector<pef::SceneState> hst(n_scenes);
+  for (uint32_t k = 0; k < n_scenes; ++k) {
+    hst[k].c = params[k].sound_speed;
+    hst[k].B = params[k].rest_density * params[k].sound_speed * params[k].sound_speed / params[k].gamma;
+    hst[k].vmax2_bits = 0;
+    hst[k].pad = 0;
+  }
+  pef::Ctl hctl;
+  std::memset(&hctl, 0, sizeof(hctl));
+  hctl.force_rebuild = 1;
+  double *dx = nullptr, *dv = nullptr, *dpsi = nullptr, *dn = nullptr;
+  uint8_t* dk = nullptr;
+  uint32_t* dscene = nullptr;
+  e = hipMalloc(&dx, 3ull * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dv, 3ull * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dn, 3ull * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dpsi, (size_t)N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dk, N);
+  if (e == hipSuccess) e = hipMalloc(&dscene, (size_t)N * 4);
+  hipStream_t st = w.stream;
+  if (e == hipSuccess) e = hipMemcpyAsync(dx, hx.data(), 3ull * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, hv.data(), 3ull * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dn, hn.data(), 3ull * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpsi, hpsi.data(), (size_t)N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dk, hk.data(), N, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dscene, hscene.data(), (size_t)N * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w.sc_d, w.scenes.data(), n_scenes * sizeof(pef::SceneDev), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w.st_d, hst.data(), n_scenes * sizeof(pef::SceneState), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(w.ctl_d, &hctl, sizeof(hctl), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(w.counts, 0, (size_t)w.C * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(w.CNT, 0, (size_t)N * 4, st);
+  if (e == hipSuccess) {
+    pef::launch_stage(st, N, w.sc_d, w.cur, dx, dv, dk, dpsi, dn, dscene);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(dx);
+  (void)hipFree(dv);
+  (void)hipFree(dn);
+  (void)hipFree(dpsi);
+  (void)hipFree(dk);
+  (void)hipFree(dscene);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "pef: upload failed: %s\n", hipGetErrorString(e));
+    return fail(PEF_ERR_PANIC);
+  }
+  w.density_valid = false;
+  w.h_psi = std::move(hpsi);
+  w.h_nout = std::move(hn);
+  w.h_kind = std::move(hk);
+  *out = pw;
+  return PEF_OK;
+}
+
+pef::GridArgs grid_args(World& w) {
+  pef::GridArgs g;
+  g.N = w.N;
+  g.C = w.C;
+  g.ctl = w.ctl_d;
+  g.sc = w.sc_d;
+  g.meta = w.cur.META;
+  g.X = w.cur.X;
+  g.Y = w.cur.Y;
+  g.Z = w.cur.Z;
+  g.cell_of = w.cell_of;
+  g.rank_of = w.rank_of;
+  g.counts = w.counts;
+  g.cell_start = w.cell_start;
+  g.block_sums = w.block_sums;
+  g.perm = w.perm;
+  return g;
+}
+
+// Timed launch helper for profiling mode.
+struct Prof {
+  World& w;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> names;
+  explicit Prof(World& ww) : w(ww) {}
+  void mark(const char* name) {
+    if (!w.profiling) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, w.stream);
+    ev.push_back(e);
+    names.push_back(name);
+  }
+  void finish() {
+    if (!w.profiling || ev.size() < 2) return;
+    (void)hipEventSynchronize(ev.back());
+    for (size_t k = 1; k < ev.size(); ++k) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+      auto it = std::find(w.prof_names.begin(), w.prof_names.end(), names[k]);
+      if (it == w.prof_names.end()) {
+        w.prof_names.push_back(names[k]);
+        w.prof_ms.push_back(ms);
+      } else {
+        w.prof_ms[it - w.prof_names.begin()] += ms;
+      }
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    names.clear();
+  }
+};
+
+void enqueue_rebuild(World& w, Prof* pr) {
+  pef::launch_decide(w.stream, w.ctl_d, w.st_d, w.S);
+  if (pr) pr->mark("decide");
+  pef::launch_grid_build(w.stream, grid_args(w), 0);
+  if (pr) pr->mark("grid_count_scan_scatter");
+  pef::launch_finalize(w.stream, w.N, w.ctl_d, w.sc_d, w.cell_of, w.cell_start, w.perm, w.cur, w.alt, w.counts, w.C);
+  if (pr) pr->mark("grid_finalize");
+  pef::launch_nlist(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.cell_start, w.cur, w.LIST, w.CNT, w.cap,
+                    w.with_boundary, 0, w.bs);
+  if (pr) pr->mark("nlist");
+}
+
+void enqueue_body(World& w, Prof* pr) {
+  if (pr) pr->mark("start");
+  pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+  if (pr) pr->mark("pressure");
+  pef::launch_force(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, w.LIST, w.CNT, w.cap, 1, w.with_boundary,
+                    w.bs);
+  if (pr) pr->mark("force");
+  enqueue_rebuild(w, pr);
+  pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  if (pr) pr->mark("density");
+  pef::launch_xsph(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  if (pr) pr->mark("xsph");
+}
+
+int32_t read_flags(World& w) {
+  pef::Ctl h;
+  HIPTRY(hipMemcpyAsync(&h, w.ctl_d, sizeof(h), hipMemcpyDeviceToHost, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  if (h.flags & pef::kFlagGrid) return PEF_ERR_GRID;
+  if (h.flags & pef::kFlagNeighbours) return PEF_ERR_NEIGHBOURS;
+  return PEF_OK;
+}
+
+int32_t set_force_rebuild(World& w) {
+  uint32_t one = 1;
+  HIPTRY(hipMemcpyAsync((char*)w.ctl_d + offsetof(pef::Ctl, force_rebuild), &one, 4, hipMemcpyHostToDevice, w.stream));
+  return PEF_OK;
+}
+
+// rebuild_grid + compute_density when the densities are not those of the
+// current positions.
+int32_t ensure_density(World& w) {
+  if (w.density_valid) return PEF_OK;
+  int32_t rc = set_force_rebuild(w);
+  if (rc) return rc;
+  Prof pr(w);
+  pr.mark("start");
+  enqueue_rebuild(w, &pr);
+  pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  pr.mark("density");
+  HIPTRY(hipGetLastError());
+  pr.finish();
+  w.density_valid = true;
+  return PEF_OK;
+}
+
+int32_t do_step(World& w, double dt, uint32_t nsteps) {
+  if (!(std::isfinite(dt) && dt > 0.0)) return PEF_ERR_ARG;
+  if (nsteps == 0) return PEF_OK;
+  HIPTRY(hipMemcpyAsync((char*)w.ctl_d + offsetof(pef::Ctl, dt), &dt, sizeof(double), hipMemcpyHostToDevice,
+                        w.stream));
+  int32_t rc = ensure_density(w);
+  if (rc) return rc;
+  w.probe_valid = false;
+  if (w.profiling || !w.use_graph) {
+    for (uint32_t k = 0; k < nsteps; ++k) {
+      Prof pr(w);
+      enqueue_body(w, w.profiling ? &pr : nullptr);
+      HIPTRY(hipGetLastError());
+      pr.finish();
+    }
+  } else {
+    if (!w.graph) {
+      hipGraph_t g;
+      HIPTRY(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+      enqueue_body(w, nullptr);
+      HIPTRY(hipStreamEndCapture(w.stream, &g));
+      hipError_t e = hipGraphInstantiate(&w.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPTRY(e);
+    }
+    for (uint32_t k = 0; k < nsteps; ++k) HIPTRY(hipGraphLaunch(w.graph, w.stream));
+  }
+  for (uint32_t k = 0; k < nsteps; ++k) w.t += dt;
+  return read_flags(w);
+}
+
+int32_t do_phase(World& w, int32_t phase) {
+  w.probe_valid = false;
+  switch (phase) {
+    case PEF_PHASE_REBUILD_GRID: {
+      int32_t rc = set_force_rebuild(w);
+      if (rc) return rc;
+      enqueue_rebuild(w, nullptr);
+      break;
+    }
+    case PEF_PHASE_DENSITY:
+      if (!w.density_valid) {
+        // lists must exist for the current positions
+        int32_t rc = set_force_rebuild(w);
+        if (rc) return rc;
+        enqueue_rebuild(w, nullptr);
+      }
+      pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+      w.density_valid = true;
+      break;
+    case PEF_PHASE_PRESSURE:
+      pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+      break;
+    case PEF_PHASE_FORCES:
+      pef::launch_force(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, w.LIST, w.CNT, w.cap, 0,
+                        w.with_boundary, w.bs);
+      break;
+    case PEF_PHASE_XSPH: {
+      // XSPH reads B: make it the current velocity first
+      HIPTRY(hipMemcpyAsync(w.cur.B, w.cur.B2, (size_t)w.N * sizeof(float4), hipMemcpyDeviceToDevice, w.stream));
+      pef::launch_xsph(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+      break;
+    }
+    default:
+      return PEF_ERR_ARG;
+  }
+  HIPTRY(hipGetLastError());
+  return read_flags(w);
+}
+
+int32_t get_field(World& w, int what, double* out, uint64_t len, bool vec) {
+  if (!out) return PEF_ERR_NULL;
+  uint64_t need = vec ? 3ull * w.N : (uint64_t)w.N;
+  if (len < need) return PEF_ERR_BOUNDS;
+  pef::launch_unsort(w.stream, w.N, w.sc_d, w.st_d, w.cur, what, w.stage);
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipMemcpyAsync(out, w.stage, need * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  return PEF_OK;
+}
+
+// Reference grid (cell = 2h, grid.rs) over the current positions, on the device.
+int32_t build_probe(World& w) {
+  if (w.probe_valid) return PEF_OK;
+  w.release_probe();
+  // key bounds per scene
+  int32_t* d_bounds = nullptr;
+  std::vector<int32_t> hb(6ull * w.S);
+  for (uint32_t k = 0; k < w.S; ++k) {
+    hb[6 * k + 0] = hb[6 * k + 1] = hb[6 * k + 2] = INT32_MAX;
+    hb[6 * k + 3] = hb[6 * k + 4] = hb[6 * k + 5] = INT32_MIN;
+  }
+  std::vector<pef::SceneDev> psc = w.scenes;
+  for (uint32_t k = 0; k < w.S; ++k) {
+    psc[k].cell = 2.0 * w.params[k].h;  // WcsphWorld::new: UniformGrid::new(support_radius) (wcsph.rs:84-85)
+    psc[k].inv_cell = 1.0 / psc[k].cell;
+  }
+  HIPTRY(hipMalloc(&w.probe_sc, w.S * sizeof(pef::SceneDev)));
+  HIPTRY(hipMemcpyAsync(w.probe_sc, psc.data(), w.S * sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
+  HIPTRY(hipMalloc(&d_bounds, hb.size() * 4));
+  HIPTRY(hipMemcpyAsync(d_bounds, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, w.stream));
+  pef::launch_key_bounds(w.stream, w.N, w.probe_sc, w.cur, d_bounds);
+  HIPTRY(hipMemcpyAsync(hb.data(), d_bounds, hb.size() * 4, hipMemcpyDeviceToHost, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  (void)hipFree(d_bounds);
+  uint64_t cells = 0;
+  for (uint32_t k = 0; k < w.S; ++k) {
+    pef::SceneDev& S = psc[k];
+    if (w.scene_count[k] == 0) hb[6 * k] = hb[6 * k + 1] = hb[6 * k + 2] = hb[6 * k + 3] = hb[6 * k + 4] = hb[6 * k + 5] = 0;
+    S.ix0 = hb[6 * k];
+    S.iy0 = hb[6 * k + 1];
+    S.iz0 = hb[6 * k + 2];
+    S.nx = hb[6 * k + 3] - hb[6 * k] + 1;
+    S.ny = hb[6 * k + 4] - hb[6 * k + 1] + 1;
+    S.nz = hb[6 * k + 5] - hb[6 * k + 2] + 1;
+    S.cell_base = (uint32_t)cells;
+    cells += (uint64_t)S.nx * S.ny * S.nz;
+    if (cells > (1ull << 31)) return PEF_ERR_BOUNDS;
+  }
+  w.probe_C = (uint32_t)cells;
+  w.probe_sc_h = psc;
+  const uint32_t C = w.probe_C, N = w.N;
+  HIPTRY(hipMemcpyAsync(w.probe_sc, psc.data(), w.S * sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
+  HIPTRY(hipMalloc(&w.probe_counts, (size_t)C * 4));
+  HIPTRY(hipMalloc(&w.probe_start, ((size_t)C + 1) * 4));
+  HIPTRY(hipMalloc(&w.probe_sums, ((size_t)C / pef::kScanTileHost + 2) * 4));
+  HIPTRY(hipMalloc(&w.probe_flag, (size_t)C * 4));
+  HIPTRY(hipMalloc(&w.probe_flagpos, ((size_t)C + 1) * 4));
+  HIPTRY(hipMalloc(&w.probe_slot_of_rank, (size_t)N * 4));
+  HIPTRY(hipMalloc(&w.probe_order, (size_t)N * 8));
+  HIPTRY(hipMalloc(&w.probe_keys, 3ull * N * 4));
+  pef::launch_fill_u32(w.stream, w.probe_counts, 0u, C);
+  pef::GridArgs g = grid_args(w);
+  g.C = C;
+  g.sc = w.probe_sc;
+  g.counts = w.probe_counts;
+  g.cell_start = w.probe_start;
+  g.block_sums = w.probe_sums;
+  pef::launch_grid_build(w.stream, g, 1);
+  pef::launch_probe_order(w.stream, N, w.probe_sc, w.cur, w.cell_of, w.probe_start, w.perm, w.probe_order,
+                          w.probe_keys, w.probe_slot_of_rank);
+  pef::launch_compact_cells(w.stream, C, w.probe_sc, w.S, w.probe_start, w.probe_flag, w.probe_flagpos, w.probe_sums,
+                            nullptr, nullptr, w.ctl_d);
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipStreamSynchronize(w.stream));
+  w.probe_valid = true;
+  return PEF_OK;
+}
+
+int32_t get_grid(World& w, int32_t* keys, uint64_t* order, int32_t* occupied, uint64_t* ranges, uint64_t cap_cells,
+                 uint64_t* n_cells) {
+  if (!n_cells) return PEF_ERR_NULL;
+  int32_t rc = build_probe(w);
+  if (rc) return rc;
+  uint32_t m = 0;
+  HIPTRY(hipMemcpyAsync(&m, w.probe_flagpos + w.probe_C, 4, hipMemcpyDeviceToHost, w.stream));
+  if (keys) HIPTRY(hipMemcpyAsync(keys, w.probe_keys, 3ull * w.N * 4, hipMemcpyDeviceToHost, w.stream));
+  if (order) HIPTRY(hipMemcpyAsync(order, w.probe_order, (size_t)w.N * 8, hipMemcpyDeviceToHost, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  *n_cells = m;
+  if (!occupied && !ranges) return PEF_OK;
+  if (cap_cells < m || !occupied || !ranges) return PEF_ERR_BOUNDS;
+  int32_t* d_occ = nullptr;
+  uint64_t* d_rng = nullptr;
+  HIPTRY(hipMalloc(&d_occ, 3ull * (m ? m : 1) * 4));
+  HIPTRY(hipMalloc(&d_rng, 2ull * (m ? m : 1) * 8));
+  pef::launch_compact_cells(w.stream, w.probe_C, w.probe_sc, w.S, w.probe_start, w.probe_flag, w.probe_flagpos,
+                            w.probe_sums, d_occ, d_rng, w.ctl_d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(occupied, d_occ, 3ull * m * 4, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ranges, d_rng, 2ull * m * 8, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_occ);
+  (void)hipFree(d_rng);
+  return hip_status(e);
+}
+
+int32_t scene_of_index(World& w, uint64_t i, uint32_t* scene) {
+  for (uint32_t k = 0; k < w.S; ++k)
+    if (i >= w.scene_first[k] && i < w.scene_first[k] + w.scene_count[k]) {
+      *scene = k;
+      return PEF_OK;
+    }
+  return PEF_ERR_BOUNDS;
+}
+
+int32_t get_neighbours(World& w, uint64_t i, uint64_t* buf, uint64_t cap, uint64_t* out_len) {
+  if (!out_len) return PEF_ERR_NULL;
+  if (i >= w.N) return PEF_ERR_BOUNDS;
+  uint32_t scene = 0;
+  int32_t rc = scene_of_index(w, i, &scene);
+  if (rc) return rc;
+  rc = build_probe(w);
+  if (rc) return rc;
+  uint64_t* d_out = nullptr;
+  uint64_t* d_len = nullptr;
+  uint64_t dcap = cap ? cap : 1;
+  HIPTRY(hipMalloc(&d_out, dcap * 8));
+  HIPTRY(hipMalloc(&d_len, 8));
+  pef::launch_probe_candidates(w.stream, w.probe_sc, scene, w.probe_keys + 3 * i, w.probe_start, w.probe_order,
+                               d_out, cap, d_len);
+  uint64_t n = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  if (e == hipSuccess && buf && n <= cap) e = hipMemcpyAsync(buf, d_out, n * 8, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_out);
+  (void)hipFree(d_len);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  *out_len = n;
+  if (!buf || n > cap) return PEF_ERR_BOUNDS;
+  return PEF_OK;
+}
+
+int32_t get_neighbour_list(World& w, uint64_t i, uint64_t* buf, uint64_t cap, uint64_t* out_len) {
+  if (!out_len) return PEF_ERR_NULL;
+  if (i >= w.N) return PEF_ERR_BOUNDS;
+  int32_t rc = ensure_density(w);  // lists exist for the current positions
+  if (rc) return rc;
+  uint32_t* d_slot = nullptr;
+  uint64_t* d_out = nullptr;
+  uint32_t* d_len = nullptr;
+  HIPTRY(hipMalloc(&d_slot, (size_t)w.N * 4));
+  HIPTRY(hipMalloc(&d_out, (size_t)w.cap * 8));
+  HIPTRY(hipMalloc(&d_len, 4));
+  pef::launch_slot_of_id(w.stream, w.N, w.cur, d_slot);
+  uint32_t slot = 0, n = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&slot, d_slot + i, 4, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  if (e == hipSuccess) {
+    pef::launch_extract_list(w.stream, slot, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, d_out, d_len);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&n, d_len, 4, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  std::vector<uint64_t> tmp(n);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(tmp.data(), d_out, (size_t)n * 8, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_slot);
+  (void)hipFree(d_out);
+  (void)hipFree(d_len);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  std::sort(tmp.begin(), tmp.end());  // presentation order: ascending original index
+  *out_len = n;
+  if (!buf || n > cap) return PEF_ERR_BOUNDS;
+  std::memcpy(buf, tmp.data(), (size_t)n * 8);
+  return PEF_OK;
+}
+
+uint64_t fnv1a_update(uint64_t h, double d) {
+  uint64_t b;
+  std::memcpy(&b, &d, 8);
+  for (int k = 0; k < 8; ++k) {
+    h ^= (b >> (8 * k)) & 0xffULL;
+    h *= 0x100000001b3ULL;
+  }
+  return h;
+}
+
+int32_t diagnostics(World& w, uint32_t scene, PefDiagnostics* out) {
+  if (!out) return PEF_ERR_NULL;
+  if (scene >= w.S) return PEF_ERR_BOUNDS;
+  int32_t rc = ensure_density(w);
+  if (rc) return rc;
+  const uint32_t first = (uint32_t)w.scene_first[scene], count = (uint32_t)w.scene_count[scene];
+  uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(1024u, (count + 255) / 256));
+  pef::DiagAcc* d_part = nullptr;
+  pef::DiagAcc* d_res = nullptr;
+  HIPTRY(hipMalloc(&d_part, nb * sizeof(pef::DiagAcc)));
+  HIPTRY(hipMalloc(&d_res, sizeof(pef::DiagAcc)));
+  pef::launch_diag(w.stream, first, count, w.sc_d, scene, w.cur, d_part, nb, d_res);
+  pef::DiagAcc a;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&a, d_res, sizeof(a), hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_part);
+  (void)hipFree(d_res);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  const double m = w.params[scene].mass;
+  out->total_mass = m * a.nfluid;
+  for (int q = 0; q < 3; ++q) out->momentum[q] = a.mom[q];
+  out->kinetic_energy = a.ke;
+  out->potential_energy = a.pe;
+  out->mean_density_error = a.nfluid > 0 ? a.derr_sum / a.nfluid : 0.0;
+  out->max_density_error = a.derr_max;
+  out->v_max = std::sqrt(a.v2max);
+  out->a_max = std::sqrt(a.a2max);
+  out->fluid_count = (uint64_t)a.nfluid;
+  // FNV-1a over the scene's fluid (x, v) in index order: a sequential hash,
+  // taken over the device state copied out.
+  std::vector<double> x(3ull * w.N), v(3ull * w.N);
+  rc = get_field(w, 0, x.data(), x.size(), true);
+  if (!rc) rc = get_field(w, 1, v.data(), v.size(), true);
+  if (rc) return rc;
+  const std::vector<uint8_t>& kind = w.h_kind;
+  uint64_t h = 0xcbf29ce484222325ULL;
+  for (uint64_t i = first; i < (uint64_t)first + count; ++i) {
+    if (kind[i] != 0) continue;
+    for (int q = 0; q < 3; ++q) h = fnv1a_update(h, x[3 * i + q]);
+    for (int q = 0; q < 3; ++q) h = fnv1a_update(h, v[3 * i + q]);
+  }
+  out->hash = h;
+  return PEF_OK;
+}
+
+int32_t set_state(World& w, const double* x, const double* v, const double* psi, const double* n_out) {
+  const uint64_t N = w.N;
+  for (uint64_t i = 0; i < N; ++i) {
+    if (x && !finite3(x + 3 * i)) return PEF_ERR_ARG;
+    if (v && !finite3(v + 3 * i)) return PEF_ERR_ARG;
+    if (psi && !std::isfinite(psi[i])) return PEF_ERR_ARG;
+    if (n_out && !finite3(n_out + 3 * i)) return PEF_ERR_ARG;
+  }
+  // current state in original order, patched, re-staged
+  std::vector<double> hx(3 * N), hv(3 * N);
+  int32_t rc = get_field(w, 0, hx.data(), hx.size(), true);
+  if (!rc) rc = get_field(w, 1, hv.data(), hv.size(), true);
+  if (rc) return rc;
+  if (x) std::memcpy(hx.data(), x, 3 * N * 8);
+  if (v) std::memcpy(hv.data(), v, 3 * N * 8);
+  if (psi) std::memcpy(w.h_psi.data(), psi, N * 8);
+  if (n_out) std::memcpy(w.h_nout.data(), n_out, 3 * N * 8);
+  std::vector<double>& hpsi = w.h_psi;
+  std::vector<double>& hn = w.h_nout;
+  std::vector<uint8_t>& kind = w.h_kind;
+  std::vector<uint32_t> hscene(N);
+  for (uint32_t k = 0; k < w.S; ++k)
+    for (uint64_t i = 0; i < w.scene_count[k]; ++i) hscene[w.scene_first[k] + i] = k;
+  bool any_oriented_change = false;
+  for (uint32_t k = 0; k < w.S; ++k) {
+    bool ori = false;
+    for (uint64_t i = w.scene_first[k]; i < w.scene_first[k] + w.scene_count[k]; ++i)
+      if (kind[i] && (hn[3 * i] * hn[3 * i] + hn[3 * i + 1] * hn[3 * i + 1] + hn[3 * i + 2] * hn[3 * i + 2]) > 0.0)
+        ori = true;
+    if ((w.scenes[k].oriented != 0) != ori) any_oriented_change = true;
+    w.scenes[k].oriented = ori ? 1 : 0;
+  }
+  double *dx = nullptr, *dv = nullptr, *dpsi = nullptr, *dn = nullptr;
+  uint8_t* dk = nullptr;
+  uint32_t* dsc = nullptr;
+  hipError_t e = hipMalloc(&dx, 3 * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dv, 3 * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dn, 3 * N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dpsi, N * 8);
+  if (e == hipSuccess) e = hipMalloc(&dk, N);
+  if (e == hipSuccess) e = hipMalloc(&dsc, N * 4);
+  hipStream_t st = w.stream;
+  if (e == hipSuccess) e = hipMemcpyAsync(dx, hx.data(), 3 * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, hv.data(), 3 * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dn, hn.data(), 3 * N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpsi, hpsi.data(), N * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dk, kind.data(), N, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dsc, hscene.data(), N * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && any_oriented_change)
+    e = hipMemcpyAsync(w.sc_d, w.scenes.data(), w.S * sizeof(pef::SceneDev), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    pef::launch_stage(st, (uint32_t)N, w.sc_d, w.cur, dx, dv, dk, dpsi, dn, dsc);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(dx);
+  (void)hipFree(dv);
+  (void)hipFree(dn);
+  (void)hipFree(dpsi);
+  (void)hipFree(dk);
+  (void)hipFree(dsc);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  w.density_valid = false;
+  w.probe_valid = false;
+  return set_force_rebuild(w);
+}
+
+template <class F>
+int32_t guard(F&& f) {
+  try {
+    return f();
+  } catch (...) {
+    return PEF_ERR_PANIC;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pef_version_string(void) { return "pef 0.1.0 (gfx950 WCSPH)"; }
+
+int32_t pef_default_params(double rest_density, double sound_speed, const double gravity[3], PefParams* out) {
+  if (!out || !gravity) return PEF_ERR_NULL;
+  std::memset(out, 0, sizeof(*out));
+  out->dim = 3;
+  out->kernel = PEF_KERNEL_CUBIC_SPLINE;
+  out->rest_density = rest_density;
+  out->mass = 1.0;
+  out->h = 0.1;
+  out->gamma = 7.0;
+  out->sound_speed = sound_speed;
+  out->visc_alpha = 0.01;
+  out->visc_beta = 0.0;
+  out->visc_eps = 0.01;
+  out->xsph_epsilon = 0.25;
+  for (int a = 0; a < 3; ++a) out->gravity[a] = gravity[a];
+  out->adapt_sound_speed = 1;
+  out->surface_tension_sigma = 0.0;
+  return PEF_OK;
+}
+
+int32_t pef_world_create(const PefParams* params, const PefParticles* particles, const PefOptions* options,
+                         PefWorld** out) {
+  return guard([&] { return create_world(1, params, particles, options, out); });
+}
+
+int32_t pef_batch_create(uint32_t n_scenes, const PefParams* params, const PefParticles* particles,
+                         const PefOptions* options, PefWorld** out) {
+  return guard([&] { return create_world(n_scenes, params, particles, options, out); });
+}
+
+void pef_world_destroy(PefWorld* world) {
+  if (!world) return;
+  try {
+    delete world;
+  } catch (...) {
+  }
+}
+
+int32_t pef_world_step(PefWorld* world, double dt, uint32_t nsteps) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return do_step(world->w, dt, nsteps); });
+}
+
+int32_t pef_world_phase(PefWorld* world, int32_t phase) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return do_phase(world->w, phase); });
+}
+
+int32_t pef_world_len(const PefWorld* world, uint64_t* n) {
+  if (!world || !n) return PEF_ERR_NULL;
+  *n = world->w.N;
+  return PEF_OK;
+}
+
+int32_t pef_world_scene_count(const PefWorld* world, uint32_t* n_scenes) {
+  if (!world || !n_scenes) return PEF_ERR_NULL;
+  *n_scenes = world->w.S;
+  return PEF_OK;
+}
+
+int32_t pef_world_scene_range(const PefWorld* world, uint32_t scene, uint64_t* first, uint64_t* count) {
+  if (!world || !first || !count) return PEF_ERR_NULL;
+  if (scene >= world->w.S) return PEF_ERR_BOUNDS;
+  *first = world->w.scene_first[scene];
+  *count = world->w.scene_count[scene];
+  return PEF_OK;
+}
+
+int32_t pef_world_time(const PefWorld* world, double* t) {
+  if (!world || !t) return PEF_ERR_NULL;
+  *t = world->w.t;
+  return PEF_OK;
+}
+
+int32_t pef_world_get_params(PefWorld* world, uint32_t scene, PefParams* out) {
+  if (!world || !out) return PEF_ERR_NULL;
+  World& w = world->w;
+  if (scene >= w.S) return PEF_ERR_BOUNDS;
+  return guard([&] {
+    pef::SceneState s;
+    HIPTRY(hipMemcpyAsync(&s, w.st_d + scene, sizeof(s), hipMemcpyDeviceToHost, w.stream));
+    HIPTRY(hipStreamSynchronize(w.stream));
+    *out = w.params[scene];
+    out->sound_speed = s.c;
+    return (int32_t)PEF_OK;
+  });
+}
+
+int32_t pef_world_set_params(PefWorld* world, uint32_t scene, const PefParams* params) {
+  if (!world || !params) return PEF_ERR_NULL;
+  World& w = world->w;
+  if (scene >= w.S) return PEF_ERR_BOUNDS;
+  int32_t rc = check_params(*params);
+  if (rc) return rc;
+  const PefParams& o = w.params[scene];
+  if (params->dim != o.dim || params->h != o.h || params->mass != o.mass || params->rest_density != o.rest_density)
+    return PEF_ERR_ARG;
+  return guard([&] {
+    w.params[scene] = *params;
+    fill_scene_physics(w.scenes[scene], *params, w.skin);
+    HIPTRY(hipMemcpyAsync(w.sc_d + scene, &w.scenes[scene], sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
+    pef::SceneState s;
+    s.c = params->sound_speed;
+    s.B = params->rest_density * params->sound_speed * params->sound_speed / params->gamma;
+    s.vmax2_bits = 0;
+    s.pad = 0;
+    HIPTRY(hipMemcpyAsync(w.st_d + scene, &s, sizeof(s), hipMemcpyHostToDevice, w.stream));
+    HIPTRY(hipStreamSynchronize(w.stream));
+    return (int32_t)PEF_OK;
+  });
+}
+
+#define PEF_GETTER(name, what, vec)                                           \
+  int32_t name(PefWorld* world, double* out, uint64_t len) {                 \
+    if (!world) return PEF_ERR_NULL;                                          \
+    return guard([&] { return get_field(world->w, what, out, len, vec); }); \
+  }
+PEF_GETTER(pef_world_get_positions, 0, true)
+PEF_GETTER(pef_world_get_velocities, 1, true)
+PEF_GETTER(pef_world_get_accelerations, 2, true)
+PEF_GETTER(pef_world_get_densities, 3, false)
+PEF_GETTER(pef_world_get_pressures, 4, false)
+PEF_GETTER(pef_world_get_boundary_force, 5, true)
+#undef PEF_GETTER
+
+int32_t pef_world_set_state(PefWorld* world, const double* x, const double* v, const double* psi,
+                            const double* n_out) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return set_state(world->w, x, v, psi, n_out); });
+}
+
+int32_t pef_world_get_grid(PefWorld* world, int32_t* keys, uint64_t* order, int32_t* occupied, uint64_t* ranges,
+                           uint64_t cap_cells, uint64_t* n_cells) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return get_grid(world->w, keys, order, occupied, ranges, cap_cells, n_cells); });
+}
+
+int32_t pef_world_get_neighbours(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap, uint64_t* out_len) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return get_neighbours(world->w, i, buf, cap, out_len); });
+}
+
+int32_t pef_world_get_neighbour_list(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap, uint64_t* out_len) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return get_neighbour_list(world->w, i, buf, cap, out_len); });
+}
+
+int32_t pef_world_diagnostics(PefWorld* world, uint32_t scene, PefDiagnostics* out) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return diagnostics(world->w, scene, out); });
+}
+
+int32_t pef_world_set_profiling(PefWorld* world, int32_t on) {
+  if (!world) return PEF_ERR_NULL;
+  world->w.profiling = on != 0;
+  world->w.prof_names.clear();
+  world->w.prof_ms.clear();
+  return PEF_OK;
+}
+
+int32_t pef_world_kernel_times(PefWorld* world, const char** names, double* ms, uint32_t cap, uint32_t* n) {
+  if (!world || !n) return PEF_ERR_NULL;
+  World& w = world->w;
+  *n = (uint32_t)w.prof_names.size();
+  if (!names || !ms || cap < *n) return PEF_ERR_BOUNDS;
+  for (uint32_t k = 0; k < *n; ++k) {
+    names[k] = w.prof_names[k].c_str();
+    ms[k] = w.prof_ms[k];
+  }
+  return PEF_OK;
+}
+
+int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds) {
+  if (!world || !mean_entries || !rebuilds) return PEF_ERR_NULL;
+  World& w = world->w;
+  return guard([&] {
+    int32_t rc = ensure_density(w);
+    if (rc) return rc;
+    HIPTRY(hipMemsetAsync((char*)w.ctl_d + offsetof(pef::Ctl, list_entries), 0, 8, w.stream));
+    pef::launch_list_stats(w.stream, w.N, w.CNT, w.ctl_d);
+    pef::Ctl h;
+    HIPTRY(hipMemcpyAsync(&h, w.ctl_d, sizeof(h), hipMemcpyDeviceToHost, w.stream));
+    HIPTRY(hipStreamSynchronize(w.stream));
+    *mean_entries = (double)h.list_entries / (double)w.N;
+    *rebuilds = h.rebuilds;
+    return (int32_t)PEF_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,497 @@
+"""Python mirror of pe-fluid's WCSPH API over the MI355X C ABI (include/pef.h).
+
+Names, argument meanings and error behaviour follow the reference's Rust API
+(crates/pe-fluid/src/{wcsph,particles,diagnostics}.rs) so tests read like the
+reference's own: ``Particles``, ``fill_box``, ``fill_box_boundary``,
+``recomputed_akinci_psi``, ``mass_calibrated_to_rest``, ``WcsphParams.water``
+and ``WcsphWorld`` with ``step``, ``rebuild_grid``, ``compute_density``,
+``compute_pressure``, ``compute_forces``, ``apply_xsph``, ``v_max``, ``a_max``.
+
+Every computation runs in libpef.so on the GPU. There is no fallback: if the
+library is missing, importing the world raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libpef.so")
+
+FLUID, BOUNDARY = 0, 1
+D2, D3 = 2, 3
+
+OK, ERR_NULL, ERR_BOUNDS, ERR_ARG, ERR_PANIC, ERR_UNSUPPORTED, ERR_GRID, ERR_NEIGHBOURS, ERR_DEVICE = (
+    0, 1, 2, 3, 4, 6, 7, 8, 9)
+_ERR_NAMES = {1: "PEF_ERR_NULL", 2: "PEF_ERR_BOUNDS", 3: "PEF_ERR_ARG", 4: "PEF_ERR_PANIC",
+              6: "PEF_ERR_UNSUPPORTED", 7: "PEF_ERR_GRID", 8: "PEF_ERR_NEIGHBOURS", 9: "PEF_ERR_DEVICE"}
+
+PHASE_REBUILD_GRID, PHASE_DENSITY, PHASE_PRESSURE, PHASE_FORCES, PHASE_XSPH = range(5)
+
+
+class PefError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        super().__init__(f"{where}: {_ERR_NAMES.get(code, code)}")
+
+
+class PefParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("kernel", C.c_int32),
+        ("rest_density", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("gamma", C.c_double),
+        ("sound_speed", C.c_double),
+        ("visc_alpha", C.c_double),
+        ("visc_beta", C.c_double),
+        ("visc_eps", C.c_double),
+        ("xsph_epsilon", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("adapt_sound_speed", C.c_int32),
+        ("surface_tension_sigma", C.c_double),
+    ]
+
+
+class PefParticles(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("x", C.c_void_p),
+        ("v", C.c_void_p),
+        ("kind", C.c_void_p),
+        ("psi", C.c_void_p),
+        ("n_out", C.c_void_p),
+    ]
+
+
+class PefOptions(C.Structure):
+    _fields_ = [
+        ("neighbour_capacity", C.c_uint32),
+        ("skin", C.c_double),
+        ("grid_margin_cells", C.c_uint32),
+        ("boundary_force", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("block_size", C.c_uint32),
+    ]
+
+
+class PefDiagnostics(C.Structure):
+    _fields_ = [
+        ("total_mass", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("kinetic_energy", C.c_double),
+        ("potential_energy", C.c_double),
+        ("mean_density_error", C.c_double),
+        ("max_density_error", C.c_double),
+        ("v_max", C.c_double),
+        ("a_max", C.c_double),
+        ("hash", C.c_uint64),
+        ("fluid_count", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libpef.so (built in-tree by physics-engine_amd/Makefile)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libpef.so not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    i32, u32, u64, d, vp = C.c_int32, C.c_uint32, C.c_uint64, C.c_double, C.c_void_p
+    P = C.POINTER
+    sig = {
+        "pef_version_string": (C.c_char_p, []),
+        "pef_default_params": (i32, [d, d, vp, P(PefParams)]),
+        "pef_world_create": (i32, [P(PefParams), P(PefParticles), vp, P(vp)]),
+        "pef_batch_create": (i32, [u32, vp, vp, vp, P(vp)]),
+        "pef_world_destroy": (None, [vp]),
+        "pef_world_step": (i32, [vp, d, u32]),
+        "pef_world_phase": (i32, [vp, i32]),
+        "pef_world_len": (i32, [vp, P(u64)]),
+        "pef_world_scene_count": (i32, [vp, P(u32)]),
+        "pef_world_scene_range": (i32, [vp, u32, P(u64), P(u64)]),
+        "pef_world_time": (i32, [vp, P(d)]),
+        "pef_world_get_params": (i32, [vp, u32, P(PefParams)]),
+        "pef_world_set_params": (i32, [vp, u32, P(PefParams)]),
+        "pef_world_get_positions": (i32, [vp, vp, u64]),
+        "pef_world_get_velocities": (i32, [vp, vp, u64]),
+        "pef_world_get_accelerations": (i32, [vp, vp, u64]),
+        "pef_world_get_densities": (i32, [vp, vp, u64]),
+        "pef_world_get_pressures": (i32, [vp, vp, u64]),
+        "pef_world_get_boundary_force": (i32, [vp, vp, u64]),
+        "pef_world_set_state": (i32, [vp, vp, vp, vp, vp]),
+        "pef_world_get_grid": (i32, [vp, vp, vp, vp, vp, u64, P(u64)]),
+        "pef_world_get_neighbours": (i32, [vp, u64, vp, u64, P(u64)]),
+        "pef_world_get_neighbour_list": (i32, [vp, u64, vp, u64, P(u64)]),
+        "pef_world_diagnostics": (i32, [vp, u32, P(PefDiagnostics)]),
+        "pef_world_set_profiling": (i32, [vp, i32]),
+        "pef_world_kernel_times": (i32, [vp, vp, vp, u32, P(u32)]),
+        "pef_world_list_stats": (i32, [vp, P(d), P(u64)]),
+        "pef_mass_from_spacing": (i32, [d, d, i32, P(d)]),
+        "pef_mass_calibrated_to_rest": (i32, [d, d, d, i32, P(d)]),
+        "pef_fill_box": (i32, [i32, vp, u64, u64, u64, d, vp, u64, P(u64)]),
+        "pef_fill_box_boundary": (i32, [i32, vp, vp, d, u64, vp, u64, P(u64)]),
+        "pef_recomputed_akinci_psi": (i32, [i32, d, u64, vp, vp, vp]),
+        "pef_jitter_positions": (i32, [u64, d, u64, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, where: str):
+    if rc != OK:
+        raise PefError(rc, where)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _vec3(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+# --- particles.rs helpers ------------------------------------------------------
+def mass_from_spacing(rest_density: float, dx: float, dim: int = D3) -> float:
+    out = C.c_double()
+    _check(lib().pef_mass_from_spacing(rest_density, dx, dim, C.byref(out)), "mass_from_spacing")
+    return out.value
+
+
+def mass_calibrated_to_rest(rest_density: float, dx: float, h: float, dim: int = D3) -> float:
+    out = C.c_double()
+    _check(lib().pef_mass_calibrated_to_rest(rest_density, dx, h, dim, C.byref(out)), "mass_calibrated_to_rest")
+    return out.value
+
+
+class Particles:
+    """SoA particle buffer (particles.rs:20-48), host side, original order."""
+
+    def __init__(self, dim: int, rest_density: float, mass: float, h: float):
+        self.dim, self.rest_density, self.mass, self.h = dim, float(rest_density), float(mass), float(h)
+        self.x = np.zeros((0, 3))
+        self.v = np.zeros((0, 3))
+        self.kind = np.zeros(0, np.uint8)
+        self.psi = np.zeros(0)
+        self.n_out = np.zeros((0, 3))
+
+    def __len__(self):
+        return len(self.kind)
+
+    def fluid_count(self) -> int:
+        return int(np.sum(self.kind == FLUID))
+
+    def _append(self, x, v, kind, psi, n_out):
+        self.x = np.concatenate([self.x, np.asarray(x, float).reshape(-1, 3)])
+        self.v = np.concatenate([self.v, np.asarray(v, float).reshape(-1, 3)])
+        self.kind = np.concatenate([self.kind, np.asarray(kind, np.uint8).reshape(-1)])
+        self.psi = np.concatenate([self.psi, np.asarray(psi, float).reshape(-1)])
+        self.n_out = np.concatenate([self.n_out, np.asarray(n_out, float).reshape(-1, 3)])
+
+    def push_fluid(self, x, v=(0.0, 0.0, 0.0)):  # particles.rs:90-99
+        self._append([x], [v], [FLUID], [0.0], [(0.0, 0.0, 0.0)])
+
+    def push_boundary(self, x, psi):  # particles.rs:102-104
+        self.push_boundary_oriented(x, psi, (0.0, 0.0, 0.0))
+
+    def push_boundary_oriented(self, x, psi, n_out):  # particles.rs:107-116
+        self._append([x], [(0.0, 0.0, 0.0)], [BOUNDARY], [psi], [n_out])
+
+    def total_fluid_mass(self) -> float:
+        return self.mass * self.fluid_count()
+
+
+def fill_box(parts: Particles, origin, nx: int, ny: int, nz: int, dx: float, v0=(0.0, 0.0, 0.0)):
+    """particles.rs:220-245 (iz -> iy -> ix)."""
+    o = _vec3(origin)
+    n = C.c_uint64()
+    lib().pef_fill_box(parts.dim, _ptr(o), nx, ny, nz, dx, None, 0, C.byref(n))
+    out = np.zeros((n.value, 3))
+    _check(lib().pef_fill_box(parts.dim, _ptr(o), nx, ny, nz, dx, _ptr(out), out.size, C.byref(n)), "fill_box")
+    parts._append(out, np.tile(np.asarray(v0, float), (n.value, 1)), np.zeros(n.value, np.uint8),
+                  np.zeros(n.value), np.zeros((n.value, 3)))
+
+
+def fill_box_boundary(parts: Particles, mn, mx, dx: float, layers: int = 1):
+    """particles.rs:286-347 (provisional psi = dx^dim)."""
+    a, b = _vec3(mn), _vec3(mx)
+    n = C.c_uint64()
+    lib().pef_fill_box_boundary(parts.dim, _ptr(a), _ptr(b), dx, layers, None, 0, C.byref(n))
+    out = np.zeros((n.value, 3))
+    _check(lib().pef_fill_box_boundary(parts.dim, _ptr(a), _ptr(b), dx, layers, _ptr(out), out.size, C.byref(n)),
+           "fill_box_boundary")
+    psi = dx * dx if parts.dim == D2 else dx * dx * dx
+    parts._append(out, np.zeros((n.value, 3)), np.ones(n.value, np.uint8), np.full(n.value, psi),
+                  np.zeros((n.value, 3)))
+
+
+def recomputed_akinci_psi(parts: Particles):
+    """particles.rs:252-279, computed on the GPU."""
+    x = np.ascontiguousarray(parts.x, dtype=np.float64)
+    k = np.ascontiguousarray(parts.kind, dtype=np.uint8)
+    psi = np.ascontiguousarray(parts.psi, dtype=np.float64).copy()
+    _check(lib().pef_recomputed_akinci_psi(parts.dim, parts.h, len(k), _ptr(x), _ptr(k), _ptr(psi)),
+           "recomputed_akinci_psi")
+    parts.psi = psi
+
+
+def jitter_positions(parts: Particles, seed: int, amp: float):
+    """Fluid positions += uniform(-amp, amp) per component from xoshiro256** seed (rng.rs:35, 70)."""
+    x = np.ascontiguousarray(parts.x, dtype=np.float64).copy()
+    k = np.ascontiguousarray(parts.kind, dtype=np.uint8)
+    _check(lib().pef_jitter_positions(seed, amp, len(k), _ptr(k), _ptr(x)), "jitter_positions")
+    parts.x = x
+
+
+# --- wcsph.rs -------------------------------------------------------------------
+@dataclass
+class WcsphParams:
+    """WcsphParams (wcsph.rs:22-39); water() = wcsph.rs:43-54."""
+
+    rest_density: float = 1000.0
+    sound_speed: float = 20.0
+    gamma: float = 7.0
+    visc_alpha: float = 0.01
+    visc_beta: float = 0.0
+    visc_eps: float = 0.01
+    xsph_epsilon: float = 0.25
+    gravity: Sequence[float] = (0.0, 0.0, -9.80665)
+    adapt_sound_speed: bool = True
+    surface_tension_sigma: float = 0.0
+    kernel: int = 0
+
+    @staticmethod
+    def water(rest_density: float, sound_speed: float, gravity) -> "WcsphParams":
+        return WcsphParams(rest_density=rest_density, sound_speed=sound_speed, gravity=tuple(gravity))
+
+    def to_c(self, parts: Particles) -> PefParams:
+        p = PefParams()
+        p.dim, p.kernel = parts.dim, self.kernel
+        p.rest_density, p.mass, p.h = parts.rest_density, parts.mass, parts.h
+        p.gamma, p.sound_speed = self.gamma, self.sound_speed
+        p.visc_alpha, p.visc_beta, p.visc_eps = self.visc_alpha, self.visc_beta, self.visc_eps
+        p.xsph_epsilon = self.xsph_epsilon
+        for a in range(3):
+            p.gravity[a] = float(self.gravity[a])
+        p.adapt_sound_speed = 1 if self.adapt_sound_speed else 0
+        p.surface_tension_sigma = self.surface_tension_sigma
+        return p
+
+
+@dataclass
+class Options:
+    neighbour_capacity: int = 0
+    skin: float = 0.0  # 0 = library default; negative = no skin (rebuild whenever anything moved)
+    grid_margin_cells: int = 0
+    boundary_force: bool = True
+    use_graph: bool = True
+    block_size: int = 0
+
+    def to_c(self) -> PefOptions:
+        o = PefOptions()
+        o.neighbour_capacity = self.neighbour_capacity
+        o.skin = self.skin
+        o.grid_margin_cells = self.grid_margin_cells
+        o.boundary_force = 1 if self.boundary_force else -1
+        o.use_graph = 1 if self.use_graph else -1
+        o.block_size = self.block_size
+        return o
+
+
+def _c_particles(parts: Particles, keep: list) -> PefParticles:
+    x = np.ascontiguousarray(parts.x, dtype=np.float64)
+    v = np.ascontiguousarray(parts.v, dtype=np.float64)
+    k = np.ascontiguousarray(parts.kind, dtype=np.uint8)
+    psi = np.ascontiguousarray(parts.psi, dtype=np.float64)
+    n = np.ascontiguousarray(parts.n_out, dtype=np.float64)
+    keep.extend([x, v, k, psi, n])
+    cp = PefParticles()
+    cp.n = len(k)
+    cp.x, cp.v, cp.kind, cp.psi, cp.n_out = _ptr(x), _ptr(v), _ptr(k), _ptr(psi), _ptr(n)
+    return cp
+
+
+class WcsphWorld:
+    """WcsphWorld (wcsph.rs:56-370) backed by the device. One or many scenes."""
+
+    def __init__(self, particles, params, options: Optional[Options] = None):
+        L = lib()
+        self._h = C.c_void_p()
+        keep: list = []
+        opts = (options or Options()).to_c()
+        if isinstance(particles, Particles):
+            self.scenes = [particles]
+            self._params = [params]
+            cp = _c_particles(particles, keep)
+            pp = params.to_c(particles)
+            _check(L.pef_world_create(C.byref(pp), C.byref(cp), C.byref(opts), C.byref(self._h)), "pef_world_create")
+        else:
+            self.scenes = list(particles)
+            self._params = list(params)
+            ns = len(self.scenes)
+            cps = (PefParticles * ns)(*[_c_particles(p, keep) for p in self.scenes])
+            pps = (PefParams * ns)(*[pr.to_c(p) for pr, p in zip(self._params, self.scenes)])
+            _check(L.pef_batch_create(ns, C.cast(pps, C.c_void_p), C.cast(cps, C.c_void_p), C.byref(opts),
+                                      C.byref(self._h)), "pef_batch_create")
+        n = C.c_uint64()
+        _check(L.pef_world_len(self._h, C.byref(n)), "pef_world_len")
+        self.n = n.value
+        self.dim = self.scenes[0].dim
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().pef_world_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    # --- stepping ---
+    def step(self, dt: float, nsteps: int = 1):
+        _check(lib().pef_world_step(self._h, dt, nsteps), "pef_world_step")
+
+    def rebuild_grid(self):
+        _check(lib().pef_world_phase(self._h, PHASE_REBUILD_GRID), "rebuild_grid")
+
+    def compute_density(self):
+        _check(lib().pef_world_phase(self._h, PHASE_DENSITY), "compute_density")
+
+    def compute_pressure(self):
+        _check(lib().pef_world_phase(self._h, PHASE_PRESSURE), "compute_pressure")
+
+    def compute_forces(self):
+        _check(lib().pef_world_phase(self._h, PHASE_FORCES), "compute_forces")
+
+    def apply_xsph(self):
+        _check(lib().pef_world_phase(self._h, PHASE_XSPH), "apply_xsph")
+
+    # --- state ---
+    @property
+    def t(self) -> float:
+        t = C.c_double()
+        _check(lib().pef_world_time(self._h, C.byref(t)), "pef_world_time")
+        return t.value
+
+    def _get(self, fn: str, vec: bool) -> np.ndarray:
+        out = np.zeros((self.n, 3)) if vec else np.zeros(self.n)
+        _check(getattr(lib(), fn)(self._h, _ptr(out), out.size), fn)
+        return out
+
+    def positions(self):
+        return self._get("pef_world_get_positions", True)
+
+    def velocities(self):
+        return self._get("pef_world_get_velocities", True)
+
+    def accelerations(self):
+        return self._get("pef_world_get_accelerations", True)
+
+    def densities(self):
+        return self._get("pef_world_get_densities", False)
+
+    def pressures(self):
+        return self._get("pef_world_get_pressures", False)
+
+    def boundary_force(self):
+        return self._get("pef_world_get_boundary_force", True)
+
+    def set_state(self, x=None, v=None, psi=None, n_out=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (x, v, psi, n_out)]
+        _check(lib().pef_world_set_state(self._h, *[_ptr(a) for a in arrs]), "pef_world_set_state")
+
+    def get_params(self, scene: int = 0) -> PefParams:
+        p = PefParams()
+        _check(lib().pef_world_get_params(self._h, scene, C.byref(p)), "pef_world_get_params")
+        return p
+
+    def set_params(self, p: PefParams, scene: int = 0):
+        _check(lib().pef_world_set_params(self._h, scene, C.byref(p)), "pef_world_set_params")
+
+    @property
+    def sound_speed(self) -> float:
+        return self.get_params(0).sound_speed
+
+    def scene_range(self, scene: int):
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(lib().pef_world_scene_range(self._h, scene, C.byref(a), C.byref(b)), "scene_range")
+        return a.value, b.value
+
+    # --- grid (grid.rs) ---
+    def grid(self):
+        m = C.c_uint64()
+        keys = np.zeros((self.n, 3), np.int32)
+        order = np.zeros(self.n, np.uint64)
+        _check(lib().pef_world_get_grid(self._h, _ptr(keys), _ptr(order), None, None, 0, C.byref(m)), "get_grid")
+        occ = np.zeros((m.value, 3), np.int32)
+        rng = np.zeros((m.value, 2), np.uint64)
+        _check(lib().pef_world_get_grid(self._h, None, None, _ptr(occ), _ptr(rng), m.value, C.byref(m)), "get_grid")
+        return dict(keys=keys, order=order, occupied=occ, ranges=rng)
+
+    def neighbours(self, i: int) -> np.ndarray:
+        n = C.c_uint64()
+        rc = lib().pef_world_get_neighbours(self._h, i, None, 0, C.byref(n))
+        if rc not in (OK, ERR_BOUNDS):
+            _check(rc, "get_neighbours")
+        out = np.zeros(max(n.value, 1), np.uint64)
+        _check(lib().pef_world_get_neighbours(self._h, i, _ptr(out), n.value, C.byref(n)), "get_neighbours")
+        return out[: n.value]
+
+    def neighbour_list(self, i: int) -> np.ndarray:
+        n = C.c_uint64()
+        out = np.zeros(4096, np.uint64)
+        _check(lib().pef_world_get_neighbour_list(self._h, i, _ptr(out), out.size, C.byref(n)), "neighbour_list")
+        return out[: n.value]
+
+    # --- diagnostics.rs ---
+    def diagnostics(self, scene: int = 0) -> dict:
+        d = PefDiagnostics()
+        _check(lib().pef_world_diagnostics(self._h, scene, C.byref(d)), "diagnostics")
+        return dict(total_mass=d.total_mass, momentum=np.array(d.momentum[:]), kinetic_energy=d.kinetic_energy,
+                    potential_energy=d.potential_energy, mean_density_error=d.mean_density_error,
+                    max_density_error=d.max_density_error, v_max=d.v_max, a_max=d.a_max, hash=d.hash,
+                    fluid_count=d.fluid_count)
+
+    def v_max(self, scene: int = 0) -> float:
+        return self.diagnostics(scene)["v_max"]
+
+    def a_max(self, scene: int = 0) -> float:
+        return self.diagnostics(scene)["a_max"]
+
+    # --- measurement ---
+    def set_profiling(self, on: bool):
+        _check(lib().pef_world_set_profiling(self._h, 1 if on else 0), "set_profiling")
+
+    def kernel_times(self) -> dict:
+        n = C.c_uint32()
+        lib().pef_world_kernel_times(self._h, None, None, 0, C.byref(n))
+        names = (C.c_char_p * max(n.value, 1))()
+        ms = (C.c_double * max(n.value, 1))()
+        _check(lib().pef_world_kernel_times(self._h, C.cast(names, C.c_void_p), C.cast(ms, C.c_void_p), n.value,
+                                            C.byref(n)), "kernel_times")
+        return {names[k].decode(): ms[k] for k in range(n.value)}
+
+    def list_stats(self):
+        m, r = C.c_double(), C.c_uint64()
+        _check(lib().pef_world_list_stats(self._h, C.byref(m), C.byref(r)), "list_stats")
+        return m.value, r.value
+
+
+def version() -> str:
+    return lib().pef_version_string().decode()
+
+
+from . import scenes  # noqa: E402,F401

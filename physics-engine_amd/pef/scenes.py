@@ -1,0 +1,87 @@
+"""Dam-break scenes of the benchmark configurations, built with the library's
+own scene functions (fill_box, jitter, fill_box_boundary, GPU Akinci psi).
+
+Every parameter the configurations leave open is fixed here once:
+  - walls are one layer of Akinci boundary particles on the tank faces
+    (fill_box_boundary + recomputed_akinci_psi), the reference's only WCSPH
+    wall model (wcsph.rs:144-149, 244-255);
+  - particle mass = mass_calibrated_to_rest(rho0, dx, h) (particles.rs:178-215);
+  - unstated WCSPH parameters keep WcsphParams::water defaults
+    (XSPH 0.25, adaptive sound speed, beta 0, eps 0.01);
+  - the fluid block starts at (dx/2, dx/2, dx/2) in the tank corner;
+  - jitter: per fluid particle in fill order, per component x, y, z,
+    uniform(-0.01 dx, 0.01 dx) from xoshiro256** seeded with `seed`.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, replace
+from typing import Tuple
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class DamBreak:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    tank: Tuple[float, float, float]
+    dt: float
+    steps: int
+    jitter: float = 0.01  # fraction of dx
+    seed: int = 0
+    h_over_dx: float = 2.0
+    rest_density: float = 1000.0
+    sound_speed: float = 20.0
+    visc_alpha: float = 0.05
+    gravity: Tuple[float, float, float] = (0.0, 0.0, -9.81)
+    dim: int = 3
+
+    @property
+    def h(self) -> float:
+        return self.h_over_dx * self.dx
+
+    @property
+    def n_fluid(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+# 8,000-particle CPU case: 20^3 block, dx 0.02, 1.0 x 0.5 x 0.5 m tank, 1,000 steps at 1e-4 s.
+CPU_8K = DamBreak("dam_break_8k", 20, 20, 20, 0.02, (1.0, 0.5, 0.5), 1e-4, 1000, jitter=0.0)
+# 65,536-particle parity scene: 64 x 32 x 32 at dx 0.005, 200 steps at 2.5e-5 s.
+CHECK_64K = DamBreak("dam_break_64k", 64, 32, 32, 0.005, (0.8, 0.16, 0.2), 2.5e-5, 200)
+# 1,048,576-particle GPU benchmark: 128 x 64 x 128 at dx 0.005, 500 steps at 2.5e-5 s.
+GPU_1M = DamBreak("dam_break_1m", 128, 64, 128, 0.005, (1.6, 0.32, 0.8), 2.5e-5, 500)
+# 8,388,608-particle large run: 256 x 128 x 256 at dx 0.005.
+GPU_8M = DamBreak("dam_break_8m", 256, 128, 256, 0.005, (3.2, 0.64, 1.6), 2.5e-5, 2000)
+
+CONFIGS = {c.name: c for c in (CPU_8K, CHECK_64K, GPU_1M, GPU_8M)}
+
+
+def small(nx=12, ny=8, nz=10, dx=0.01, steps=20, jitter=0.01, dim=3) -> DamBreak:
+    """A small dam break for quick parity runs (tank 2.5x the block in x)."""
+    tank = (round(2.5 * nx * dx, 10), round(ny * dx + 2 * dx, 10), round(1.25 * nz * dx + 2 * dx, 10))
+    if dim == 2:
+        tank = (tank[0], tank[2], 0.0)
+    return DamBreak(f"small_{nx}x{ny}x{nz}", nx, ny, nz, dx, tank, 2.5e-5 * dx / 0.005, steps, jitter=jitter,
+                    dim=dim, gravity=(0.0, -9.81, 0.0) if dim == 2 else (0.0, 0.0, -9.81))
+
+
+def build(cfg: DamBreak):
+    """Particles and WcsphParams of a dam break, built through libpef."""
+    import pef
+
+    m = pef.mass_calibrated_to_rest(cfg.rest_density, cfg.dx, cfg.h, cfg.dim)
+    parts = pef.Particles(cfg.dim, cfg.rest_density, m, cfg.h)
+    o = 0.5 * cfg.dx
+    origin = (o, o, o) if cfg.dim == 3 else (o, o, 0.0)
+    pef.fill_box(parts, origin, cfg.nx, cfg.ny, cfg.nz, cfg.dx)
+    if cfg.jitter:
+        pef.jitter_positions(parts, cfg.seed, cfg.jitter * cfg.dx)
+    pef.fill_box_boundary(parts, (0.0, 0.0, 0.0), cfg.tank, cfg.dx, 1)
+    pef.recomputed_akinci_psi(parts)
+    params = pef.WcsphParams.water(cfg.rest_density, cfg.sound_speed, cfg.gravity)
+    params.visc_alpha = cfg.visc_alpha
+    return parts, params

@@ -1,0 +1,259 @@
+"""GPU parity of the WCSPH path against the f64 oracle, through the C ABI.
+
+Bars (stated per test): grid keys, order, occupied cells, ranges and the
+unfiltered candidate lists are bit-exact on identical positions; the device's
+filtered neighbour sets equal the oracle's r < 2h sets (pairs within 1e-6 h of
+the support edge, where W and grad W vanish, are excluded); Akinci psi is
+bit-exact; densities agree to RMS relative 1e-5 (FP32 with a compensated sum);
+accelerations, boundary reactions and velocities agree to the FP32 tolerances
+written in each test.
+"""
+import numpy as np
+import pytest
+
+import pef
+import pyoracle as O
+from pef import scenes
+from helpers import gpu_world, oracle_scene, oracle_world, rel_rms, vec_rel_rms
+
+pytestmark = pytest.mark.gpu
+
+SMALL = scenes.small(12, 8, 10, dx=0.01, steps=20)
+MED = scenes.small(32, 16, 16, dx=0.005, steps=10)
+
+
+def test_akinci_psi_bit_exact():
+    cfg = scenes.CHECK_64K
+    parts, _ = scenes.build(cfg)
+    _, a = oracle_scene(cfg)
+    assert np.array_equal(parts.x.view(np.uint64), a["x"].view(np.uint64))
+    assert np.array_equal(parts.psi.view(np.uint64), a["psi"].view(np.uint64))
+    assert parts.psi[parts.kind == 1].min() > 0
+
+
+@pytest.mark.parametrize("cfg", [SMALL, MED], ids=["small", "med"])
+def test_grid_and_candidates_bit_exact(cfg):
+    W, parts = gpu_world(cfg)
+    Wo, a = oracle_world(cfg)
+    Wo.rebuild_grid()
+    g, go = W.grid(), Wo.grid()
+    for k in ("keys", "order", "occupied", "ranges"):
+        assert np.array_equal(g[k].astype(np.int64), go[k].astype(np.int64)), k
+    rng = np.random.default_rng(0)
+    for i in list(rng.choice(W.n, 24, replace=False)) + [0, W.n - 1]:
+        assert np.array_equal(W.neighbours(int(i)).astype(np.int64), Wo.neighbours(int(i)).astype(np.int64))
+
+
+def _support_set(x, kind, i, h, band=1e-6):
+    r = np.linalg.norm(x - x[i], axis=1)
+    sup = 2 * h
+    inside = (r < sup - band * h)
+    edge = np.abs(r - sup) <= band * h
+    inside[i] = False
+    if kind[i] == 1:
+        inside &= kind == 0
+        edge &= kind == 0
+    return set(np.nonzero(inside)[0]), set(np.nonzero(edge)[0])
+
+
+def test_filtered_neighbour_sets_match_oracle_after_steps():
+    cfg = MED
+    W, parts = gpu_world(cfg)
+    W.step(cfg.dt, 10)
+    x = W.positions()
+    kind = parts.kind
+    rng = np.random.default_rng(1)
+    fl = np.nonzero(kind == 0)[0]
+    bd = np.nonzero(kind == 1)[0]
+    for i in list(rng.choice(fl, 40, replace=False)) + list(rng.choice(bd, 10, replace=False)):
+        got = set(W.neighbour_list(int(i)).astype(np.int64))
+        want, edge = _support_set(x, kind, int(i), cfg.h)
+        assert want <= got, (i, sorted(want - got)[:5])
+        assert got - want <= edge, (i, sorted(got - want - edge)[:5])
+    # the grid of the stepped positions is still the reference's grid
+    Wo, _ = oracle_world(cfg)
+    Wo.set_state(x=x)
+    Wo.rebuild_grid()
+    g, go = W.grid(), Wo.grid()
+    for k in ("keys", "order", "occupied", "ranges"):
+        assert np.array_equal(g[k].astype(np.int64), go[k].astype(np.int64)), k
+
+
+def test_phases_match_oracle_on_identical_state():
+    # tests/fluid_analytic.rs:154-158 drives the sub-phases one by one.
+    cfg = MED
+    W, parts = gpu_world(cfg)
+    Wo, a = oracle_world(cfg)
+    fl = a["kind"] == 0
+    rng = np.random.default_rng(2)
+    v = np.zeros((W.n, 3))
+    v[fl] = rng.normal(0, 0.2, size=(fl.sum(), 3))
+    W.set_state(v=v)
+    Wo.set_state(v=v)
+    for w in (W, Wo):
+        w.rebuild_grid()
+        w.compute_density()
+        w.compute_pressure()
+        w.compute_forces()
+    s = Wo.state()
+    rho = W.densities()
+    assert rel_rms(rho[fl], s["rho"][fl]) <= 1e-6
+    assert np.max(np.abs(rho[fl] / s["rho"][fl] - 1)) <= 1e-5
+    assert np.array_equal(rho[~fl], s["rho"][~fl])
+    assert abs(W.sound_speed - Wo.sound_speed) <= 1e-12 * Wo.sound_speed
+    p = W.pressures()
+    scale = np.abs(s["p"][fl]).max()
+    assert np.sqrt(np.mean((p[fl] - s["p"][fl]) ** 2)) <= 1e-4 * scale
+    acc = W.accelerations()
+    assert vec_rel_rms(acc, s["a"], fl) <= 1e-4
+    bf = W.boundary_force()
+    bmask = (~fl) & (np.linalg.norm(s["boundary_force"], axis=1) > 0)
+    assert bmask.sum() > 0
+    assert vec_rel_rms(bf, s["boundary_force"], bmask) <= 1e-4
+    W.apply_xsph()
+    Wo.apply_xsph()
+    vg, vo = W.velocities(), Wo.state()["v"]
+    assert vec_rel_rms(vg, vo, fl) <= 1e-6
+
+
+def test_trajectory_density_rms_within_1e5():
+    cfg = MED
+    W, parts = gpu_world(cfg)
+    Wo, a = oracle_world(cfg, threads=4)
+    fl = a["kind"] == 0
+    worst = 0.0
+    for k in range(cfg.steps):
+        W.step(cfg.dt)
+        Wo.step(cfg.dt)
+        e = rel_rms(W.densities()[fl], Wo.state()["rho"][fl])
+        worst = max(worst, e)
+    assert worst <= 1e-5, worst
+    s = Wo.state()
+    dx = np.max(np.abs(W.positions() - s["x"]))
+    assert dx <= 1e-6 * cfg.h
+    assert abs(W.t - Wo.t) == 0.0
+    assert abs(W.sound_speed - Wo.sound_speed) <= 1e-6 * Wo.sound_speed
+
+
+def test_graph_replay_matches_eager_bitwise_and_is_deterministic():
+    cfg = SMALL
+    outs = []
+    for graph in (True, False, True):
+        W, _ = gpu_world(cfg, pef.Options(use_graph=graph))
+        W.step(cfg.dt, 15)
+        outs.append((W.positions(), W.velocities(), W.densities()))
+    for k in range(3):
+        assert np.array_equal(outs[0][k].view(np.uint64), outs[1][k].view(np.uint64))
+        assert np.array_equal(outs[0][k].view(np.uint64), outs[2][k].view(np.uint64))
+
+
+def test_skin_does_not_change_the_pair_sets():
+    # Rebuilding lists every step (no skin) and with a skin give the same
+    # physics up to FP32 summation order.
+    cfg = MED
+    Wa, _ = gpu_world(cfg, pef.Options(skin=-1.0))
+    Wb, parts = gpu_world(cfg, pef.Options(skin=0.15))
+    Wa.step(cfg.dt, 10)
+    Wb.step(cfg.dt, 10)
+    fl = parts.kind == 0
+    assert rel_rms(Wa.densities()[fl], Wb.densities()[fl]) <= 1e-6
+    _, ra = Wa.list_stats()
+    _, rb = Wb.list_stats()
+    assert ra > rb
+
+
+def test_batch_equals_single_worlds():
+    cfgs = [scenes.small(10, 6, 8, dx=0.01), scenes.small(14, 8, 9, dx=0.01, jitter=0.02),
+            scenes.small(6, 6, 6, dx=0.01)]
+    built = [scenes.build(c) for c in cfgs]
+    B = pef.WcsphWorld([p for p, _ in built], [q for _, q in built])
+    B.step(cfgs[0].dt, 12)
+    xb, vb = B.positions(), B.velocities()
+    for k, (p, q) in enumerate(built):
+        W = pef.WcsphWorld(p, q)
+        W.step(cfgs[0].dt, 12)
+        a, n = B.scene_range(k)
+        assert np.array_equal(xb[a:a + n].view(np.uint64), W.positions().view(np.uint64))
+        assert np.array_equal(vb[a:a + n].view(np.uint64), W.velocities().view(np.uint64))
+        db, dw = B.diagnostics(k), W.diagnostics()
+        assert db["hash"] == dw["hash"]
+
+
+def test_reference_kat_mass_conserved_2d():
+    # tests/fluid_analytic.rs:103-137 on the device.
+    dx, rho0 = 0.05, 1000.0
+    h = 2.4 * dx
+    m = pef.mass_from_spacing(rho0, dx, 2)
+    P = pef.Particles(2, rho0, m, h)
+    pef.fill_box(P, (0.1, 0.1, 0.0), 6, 6, 1, dx)
+    pef.fill_box_boundary(P, (0, 0, 0), (0.5, 0.5, 0), dx, 1)
+    prm = pef.WcsphParams.water(rho0, 40.0, (0.0, -9.80665, 0.0))
+    prm.adapt_sound_speed = False
+    prm.visc_alpha = 0.05
+    W = pef.WcsphWorld(P, prm)
+    m0 = W.diagnostics()["total_mass"]
+    W.step(0.0002, 30)
+    d = W.diagnostics()
+    assert abs(d["total_mass"] - m0) < 1e-12
+    assert np.all(W.positions()[:, 2] == 0.0)
+    # and against the oracle's trajectory
+    Wo = O.World(dict(dim=2, rest_density=rho0, mass=m, h=h, sound_speed=40.0, gravity=(0.0, -9.80665, 0.0),
+                      adapt_sound_speed=False, visc_alpha=0.05), P.x, P.kind, psi=P.psi)
+    Wo.step(0.0002, 30)
+    fl = P.kind == 0
+    assert rel_rms(W.densities()[fl], Wo.state()["rho"][fl]) <= 1e-5
+
+
+def test_reference_kat_pair_momentum_and_cloud_momentum():
+    # tests/fluid_analytic.rs:139-169, plus a 2,000-particle cloud.
+    dx, rho0 = 0.1, 1000.0
+    h = 2.2 * dx
+    m = pef.mass_from_spacing(rho0, dx, 3)
+    P = pef.Particles(3, rho0, m, h)
+    P.push_fluid((0.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    P.push_fluid((dx, 0.0, 0.0), (-0.5, 0.2, 0.0))
+    prm = pef.WcsphParams.water(rho0, 50.0, (0, 0, 0))
+    prm.adapt_sound_speed = False
+    prm.xsph_epsilon = 0.0
+    W = pef.WcsphWorld(P, prm)
+    W.rebuild_grid()
+    W.compute_density()
+    W.compute_pressure()
+    W.compute_forces()
+    assert np.linalg.norm((W.accelerations() * m).sum(0)) < 1e-8
+    rng = np.random.default_rng(5)
+    Q = pef.Particles(3, 1000.0, 0.001, 0.02)
+    for x, v in zip(rng.uniform(0, 0.2, size=(2000, 3)), rng.normal(0, 0.3, size=(2000, 3))):
+        Q.push_fluid(x, v)
+    prm = pef.WcsphParams.water(1000.0, 20.0, (0, 0, 0))
+    prm.visc_alpha, prm.visc_beta = 0.1, 0.2
+    W = pef.WcsphWorld(Q, prm)
+    W.rebuild_grid()
+    W.compute_density()
+    W.compute_pressure()
+    W.compute_forces()
+    a = W.accelerations()
+    assert np.linalg.norm(a.sum(0)) <= 1e-5 * np.abs(a).sum()
+
+
+def test_errors_and_bounds():
+    W, parts = gpu_world(SMALL)
+    with pytest.raises(pef.PefError) as e:
+        W.step(0.0)
+    assert e.value.code == pef.ERR_ARG
+    with pytest.raises(pef.PefError) as e:
+        W.step(float("nan"))
+    assert e.value.code == pef.ERR_ARG
+    import ctypes as C
+    buf = np.zeros(5)
+    assert pef.lib().pef_world_get_positions(W._h, buf.ctypes.data_as(C.c_void_p), 5) == pef.ERR_BOUNDS
+    with pytest.raises(pef.PefError) as e:
+        W.set_state(x=np.full((W.n, 3), np.nan))
+    assert e.value.code == pef.ERR_ARG
+    # a particle thrown far outside the device grid is reported, not mis-binned
+    x = W.positions()
+    x[0] = [50.0, 50.0, 50.0]
+    W.set_state(x=x)
+    with pytest.raises(pef.PefError) as e:
+        W.step(SMALL.dt)
+    assert e.value.code == pef.ERR_GRID
