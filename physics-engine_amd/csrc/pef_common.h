@@ -3,9 +3,11 @@
 // Device data layout (all arrays indexed by "slot", the particle's position in
 // the current cell-sorted order; scenes occupy fixed contiguous slot ranges):
 //   X, Y, Z   f64  positions: the long-run position accumulator
-//   A         f32x4 (x, y, z, w): position relative to the corner of the cell
-//             the particle was binned into at the last list build; w = +m for
-//             fluid, -rho0*psi for boundary (sign bit = boundary)
+//   A         f32x4 (x, y, z, w): x, y, z hold int32 position units (the
+//             scene's unit ~ 2 nm, relative to the scene's frame origin), so a
+//             pair separation is an exact integer difference converted to FP32
+//             exactly; w = +m for fluid, -rho0*psi for boundary (sign bit =
+//             boundary)
 //   A2        f32x4 same positions, w = rho for fluid (written by density)
 //   B         f32x4 (vx, vy, vz, -) velocities
 //   B2        f32x4 (vx, vy, vz, P) velocities after XSPH, P = p / rho^2
@@ -13,10 +15,10 @@
 //   META      u32  bit0 boundary, bit1 oriented boundary, bits 8.. scene index
 //   CELL      u32  global id of the cell at the last list build
 //   DISP      f32  displacement bound since the last list build
-//   LIST      u32  neighbour entries, ELL by 64-slot chunk:
-//             LIST[(chunk * cap + k) * 64 + lane]; entry = slot | cell-offset
-//             code << 26 (2-bit signed dx, dy, dz of the neighbour's build cell
-//             relative to the particle's build cell)
+//   LIST      u32  neighbour slots, blocked ELL by 64-slot chunk:
+//             LIST[((chunk * cap/4 + k/4) * 64 + lane) * 4 + k%4]; each list is
+//             in ascending slot order and padded to a multiple of 4 with the
+//             sentinel slot N
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -64,6 +66,14 @@ struct SceneDev {
   double h_d;
   double rho0_d;
   double gamma_d;
+  // integer position frame: x = org + unit * i
+  double org[3];
+  double unit_d, inv_unit;
+  float q_unit;      // q = r / h per position unit
+  float r2s_u;       // (2h / unit)^2
+  float r2l_u;       // (2h (1 + skin) / unit)^2
+  float h_u;         // h / unit
+  float eh2_u;       // eps h^2 / unit^2
 };
 
 // Mutable per-scene state on the device.
@@ -114,22 +124,37 @@ __device__ __host__ inline int32_t floor_i32(double x) {
   return t;
 }
 
+// Position units: |units| <= 2^30 covers twice the scene's grid extent; the
+// sentinel sits at 1.5 * 2^30, so its difference with any real particle is at
+// least 2^29 units and never overflows int32.
+constexpr int32_t kSentinelUnits = 1610612736;
+__device__ __host__ inline int32_t to_units(double x, double org, double inv_unit) {
+  double u = (x - org) * inv_unit;
+  u = u < -1073741824.0 ? -1073741824.0 : (u > 1073741824.0 ? 1073741824.0 : u);
+  return (int32_t)__builtin_rint(u);
+}
+__device__ inline float units_diff(float a, float b) {
+  return (float)(__float_as_int(a) - __float_as_int(b));  // exact: |difference| < 2^24 for neighbours
+}
+
 __device__ inline uint32_t scene_of(uint32_t meta) { return meta >> kMetaSceneShift; }
 __device__ inline bool is_boundary_w(float w) { return (__float_as_uint(w) >> 31) != 0u; }
 
 // Cubic spline W and dW/dr in FP32 (kernels.rs:79-116); q < 2 is the caller's
 // guarantee (pairs are filtered on r^2 < (2h)^2 first).
+// Both branches are evaluated and selected, so a wave whose lanes straddle
+// q = 1 does not execute two code paths.
 __device__ inline float cubic_w(float q, float alpha) {
-  float t = 2.0f - q;
+  float t = fmaxf(2.0f - q, 0.0f);
   float inner = 1.0f - q * q * (1.5f - 0.75f * q);
   float outer = 0.25f * t * t * t;
-  return alpha * (q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f));
+  return alpha * (q < 1.0f ? inner : outer);
 }
 __device__ inline float cubic_dwdr(float q, float alpha_over_h) {
-  float t = 2.0f - q;
+  float t = fmaxf(2.0f - q, 0.0f);
   float inner = q * (-3.0f + 2.25f * q);
   float outer = -0.75f * t * t;
-  return alpha_over_h * (q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f));
+  return alpha_over_h * (q < 1.0f ? inner : outer);
 }
 
 // (1 + d)^n - 1 for integer n >= 1 by the binomial expansion in Horner form;

@@ -189,24 +189,19 @@ __global__ void k_finalize(uint32_t N, const Ctl* __restrict__ ctl, const SceneD
   uint32_t dst = ranked_dst(s, c, my_id, cell_start, perm, cur.ID);
   uint32_t meta = cur.META[s];
   const SceneDev& S = sc[scene_of(meta)];
-  uint32_t local = c - S.cell_base;
-  uint32_t lz = local % (uint32_t)S.nz;
-  uint32_t t = local / (uint32_t)S.nz;
-  uint32_t ly = t % (uint32_t)S.ny;
-  uint32_t lx = t / (uint32_t)S.ny;
   double x = cur.X[s], y = cur.Y[s], z = cur.Z[s];
-  double cx = (double)(S.ix0 + (int32_t)lx) * (double)S.cell_f;
-  double cy = (double)(S.iy0 + (int32_t)ly) * (double)S.cell_f;
-  double cz = (double)(S.iz0 + (int32_t)lz) * (double)S.cell_f;
   float w = (meta & kMetaBoundary) ? cur.A[s].w : S.mass;
+  const float ux = __int_as_float(to_units(x, S.org[0], S.inv_unit));
+  const float uy = __int_as_float(to_units(y, S.org[1], S.inv_unit));
+  const float uz = __int_as_float(to_units(z, S.org[2], S.inv_unit));
   alt.X[dst] = x;
   alt.Y[dst] = y;
   alt.Z[dst] = z;
-  alt.A[dst] = make_float4((float)(x - cx), (float)(y - cy), (float)(z - cz), w);
+  alt.A[dst] = make_float4(ux, uy, uz, w);
   alt.B[dst] = cur.B[s];
   alt.B2[dst] = cur.B2[s];
-  float4 a2 = cur.A2[s];
-  alt.A2[dst] = make_float4(0.0f, 0.0f, 0.0f, a2.w);  // density and pressure travel with the particle
+  // density and pressure travel with the particle
+  alt.A2[dst] = make_float4(ux, uy, uz, cur.A2[s].w);
   alt.ID[dst] = my_id;
   alt.META[dst] = meta;
   alt.CELL[dst] = c;
@@ -243,6 +238,59 @@ __global__ void k_copyback(uint32_t N, const Ctl* __restrict__ ctl, State alt, S
   }
 }
 
+// Blocked ELL list layout: the entries of a 64-slot chunk are stored four per
+// lane, LIST[((chunk * cap/4 + k/4) * 64 + lane) * 4 + k%4], so one 16-byte load
+// gives a lane four consecutive entries and a wave reads 1 KiB contiguous.
+__device__ inline uint64_t list_index(uint32_t s, uint32_t k, uint32_t cap) {
+  return ((((uint64_t)(s >> 6) * (cap >> 2) + (k >> 2)) * kWave + (s & 63u)) << 2) + (k & 3u);
+}
+__device__ inline const uint4* list_quads(const uint32_t* LIST, uint32_t s, uint32_t cap) {
+  return reinterpret_cast<const uint4*>(LIST) + (uint64_t)(s >> 6) * (cap >> 2) * kWave + (s & 63u);
+}
+// The list is streamed once per pass: load it non-temporally so it does not
+// evict the neighbour state the gathers reuse from L2.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline uint4 load_quad(const uint4* lq, uint32_t k) {
+  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(lq + (uint64_t)(k >> 2) * kWave));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Software-pipelined walk over a particle's list: while quad k (four entries)
+// is evaluated, the neighbour gathers of quad k+1 and the list load of quad
+// k+2 are already in flight. Lists are padded to a multiple of four with the
+// sentinel slot (index N: a particle far outside every support radius), so
+// every entry of a quad is a valid gather and contributes nothing when it is
+// padding. `gather(j)` loads the neighbour state of slot j;
+// `body(entry, state)` evaluates one pair.
+template <class T, class Gather, class Body>
+__device__ inline void walk_list(const uint4* lq, uint32_t n, uint32_t sentinel, Gather gather, Body body) {
+  if (n == 0) return;
+  const uint4 pad = make_uint4(sentinel, sentinel, sentinel, sentinel);
+  uint4 e0 = load_quad(lq, 0);
+  uint4 e1 = (n > 4) ? load_quad(lq, 4) : pad;
+  T g0 = gather(e0.x & kSlotMask), g1 = gather(e0.y & kSlotMask), g2 = gather(e0.z & kSlotMask),
+    g3 = gather(e0.w & kSlotMask);
+  for (uint32_t k = 0; k < n; k += 4) {
+    T h0 = gather(e1.x & kSlotMask), h1 = gather(e1.y & kSlotMask), h2 = gather(e1.z & kSlotMask),
+      h3 = gather(e1.w & kSlotMask);
+    const uint4 e2 = (k + 8 < n) ? load_quad(lq, k + 8) : pad;
+    body(e0.x, g0);
+    body(e0.y, g1);
+    body(e0.z, g2);
+    body(e0.w, g3);
+    e0 = e1;
+    e1 = e2;
+    g0 = h0;
+    g1 = h1;
+    g2 = h2;
+    g3 = h3;
+  }
+}
+
+struct Nb2 {
+  float4 a, b;
+};
+
 // ---------------------------------------------------------------------------
 // neighbour-list build (grid.rs:122-140 candidates, filtered to the list radius)
 // ---------------------------------------------------------------------------
@@ -271,7 +319,6 @@ __global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, cons
     const SceneDev& S = sc[scene_of(meta)];
     const bool i_bnd = (meta & kMetaBoundary) != 0u;
     if (!i_bnd || with_boundary) {
-      uint64_t base = ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
       float4 ai = cur.A[s];
       uint32_t c = cur.CELL[s];
       uint32_t local = c - S.cell_base;
@@ -279,27 +326,26 @@ __global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, cons
       uint32_t t = local / (uint32_t)S.nz;
       int32_t ly = (int32_t)(t % (uint32_t)S.ny);
       int32_t lx = (int32_t)(t / (uint32_t)S.ny);
-      const float L = S.cell_f;
-      const float r2l = S.r2_list;
+      const float r2l = S.r2l_u;
       const bool i_or = (meta & kMetaOriented) != 0u;
+      // Stencil cells in ascending cell id (dx outer, dz inner), so each list
+      // is in ascending slot order and the k-th entries of neighbouring lanes
+      // stay close in memory when the pair kernels gather them.
       const int zlo = (S.dim == 2) ? 0 : -1, zhi = (S.dim == 2) ? 0 : 1;
-      for (int dz = zlo; dz <= zhi; ++dz) {
-        int32_t z = lz + dz;
-        if (z < 0 || z >= S.nz) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int32_t x = lx + dx;
+        if (x < 0 || x >= S.nx) continue;
         for (int dy = -1; dy <= 1; ++dy) {
           int32_t y = ly + dy;
           if (y < 0 || y >= S.ny) continue;
-          for (int dx = -1; dx <= 1; ++dx) {
-            int32_t x = lx + dx;
-            if (x < 0 || x >= S.nx) continue;
+          for (int dz = zlo; dz <= zhi; ++dz) {
+            int32_t z = lz + dz;
+            if (z < 0 || z >= S.nz) continue;
             uint32_t cc = S.cell_base + ((uint32_t)x * (uint32_t)S.ny + (uint32_t)y) * (uint32_t)S.nz + (uint32_t)z;
             uint32_t b = cell_start[cc], e = cell_start[cc + 1];
-            // position of particle i in the frame of cell cc
-            float px = ai.x - (float)dx * L, py = ai.y - (float)dy * L, pz = ai.z - (float)dz * L;
-            uint32_t code = ((uint32_t)(dx & 3) << 26) | ((uint32_t)(dy & 3) << 28) | ((uint32_t)(dz & 3) << 30);
             for (uint32_t j = b; j < e; ++j) {
               float4 aj = cur.A[j];
-              float rx = aj.x - px, ry = aj.y - py, rz = aj.z - pz;
+              float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
               float r2 = rx * rx + ry * ry + rz * rz;
               if (!(r2 < r2l) || j == s) continue;
               const bool j_bnd = is_boundary_w(aj.w);
@@ -316,14 +362,16 @@ __global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, cons
                                 cur.Z[j], S.h_d);
               }
               if (!ok) continue;
-              if (cnt < cap) LIST[base + (uint64_t)cnt * kWave] = j | code;
+              if (cnt < cap) LIST[list_index(s, cnt, cap)] = j;
               ++cnt;
             }
           }
         }
       }
     }
-    CNT[s] = cnt < cap ? cnt : cap;
+    const uint32_t stored = cnt < cap ? cnt : cap;
+    for (uint32_t k = stored; k < ((stored + 3u) & ~3u); ++k) LIST[list_index(s, k, cap)] = N;  // sentinel
+    CNT[s] = stored;
   }
   // wave max of the counts (all lanes present), one atomic per wave
   uint32_t mx = cnt;
@@ -337,12 +385,6 @@ __global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, cons
   }
 }
 
-// offset of the neighbour's build cell relative to the particle's, in metres
-__device__ inline void entry_offset(uint32_t e, float L, float& ox, float& oy, float& oz) {
-  ox = (float)__builtin_amdgcn_sbfe(e, 26, 2) * L;
-  oy = (float)__builtin_amdgcn_sbfe(e, 28, 2) * L;
-  oz = (float)__builtin_amdgcn_sbfe(e, 30, 2) * L;
-}
 
 // ---------------------------------------------------------------------------
 // density summation with a compensated FP32 sum (wcsph.rs:117-154)
@@ -359,27 +401,28 @@ __global__ void k_density(uint32_t N, const SceneDev* __restrict__ sc, State cur
     return;
   }
   const SceneDev& S = sc[scene_of(cur.META[s])];
-  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, alpha = S.alpha;
-  float sum = S.mass * alpha;  // self term m W(0)
+  const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha;
+  // sum of weight * W / alpha (alpha is applied once at the end); self term m W(0)
+  float sum = S.mass;
   float comp = 0.0f;
-  uint32_t n = CNT[s];
-  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t e = lp[(uint64_t)k * kWave];
-    float4 aj = cur.A[e & kSlotMask];
-    float ox, oy, oz;
-    entry_offset(e, L, ox, oy, oz);
-    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
-    float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 < r2s) {
-      float r = __builtin_amdgcn_sqrtf(r2);
-      float term = fabsf(aj.w) * cubic_w(r * inv_h, alpha);
-      float y = term - comp;
-      float tt = sum + y;
-      comp = (tt - sum) - y;
-      sum = tt;
-    }
-  }
+  const uint32_t n = CNT[s];
+  const uint4* lq = list_quads(LIST, s, cap);
+  const float4* __restrict__ Ap = cur.A;
+  walk_list<float4>(
+      lq, n, N, [&](uint32_t j) { return Ap[j]; },
+      [&](uint32_t e, const float4& aj) {
+        (void)e;
+        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float r = __builtin_amdgcn_sqrtf(r2);
+        float w = fabsf(aj.w) * cubic_w(r * qu, 1.0f);
+        float term = (r2 < r2s) ? w : 0.0f;
+        float y = term - comp;  // compensated (Kahan) sum
+        float tt = sum + y;
+        comp = (tt - sum) - y;
+        sum = tt;
+      });
+  sum *= alpha;
   float rho = fmaxf(sum, 1e-12f);
   cur.A2[s] = make_float4(ai.x, ai.y, ai.z, rho);
 }
@@ -405,29 +448,29 @@ __global__ void k_xsph(uint32_t N, const SceneDev* __restrict__ sc, State cur, c
     cur.B2[s] = make_float4(bi.x, bi.y, bi.z, Pold);
     return;
   }
-  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, alpha = S.alpha, m = S.mass;
+  const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha, m = S.mass;
   float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
-  uint32_t n = CNT[s];
-  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t e = lp[(uint64_t)k * kWave];
-    uint32_t j = e & kSlotMask;
-    float4 aj = cur.A2[j];
-    if (is_boundary_w(aj.w)) continue;
-    float ox, oy, oz;
-    entry_offset(e, L, ox, oy, oz);
-    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
-    float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 < r2s) {
-      float4 bj = cur.B[j];
-      float r = __builtin_amdgcn_sqrtf(r2);
-      float wij = cubic_w(r * inv_h, alpha);
-      float sfac = m * __builtin_amdgcn_rcpf(0.5f * (ai.w + aj.w)) * wij;
-      dvx += (bj.x - bi.x) * sfac;
-      dvy += (bj.y - bi.y) * sfac;
-      dvz += (bj.z - bi.z) * sfac;
-    }
-  }
+  const uint32_t n = CNT[s];
+  const uint4* lq = list_quads(LIST, s, cap);
+  const float4* __restrict__ A2p = cur.A2;
+  const float4* __restrict__ Bp = cur.B;
+  const float two_m = 2.0f * m;
+  walk_list<Nb2>(
+      lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], Bp[j]}; },
+      [&](uint32_t e, const Nb2& nb) {
+        const float4 aj = nb.a, bj = nb.b;
+        (void)e;
+        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float r = __builtin_amdgcn_sqrtf(r2);
+        float wij = cubic_w(r * qu, alpha);
+        // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2
+        float sfac = two_m * wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
+        sfac = (r2 < r2s && aj.w > 0.0f) ? sfac : 0.0f;  // fluid neighbours only
+        dvx += (bj.x - bi.x) * sfac;
+        dvy += (bj.y - bi.y) * sfac;
+        dvz += (bj.z - bi.z) * sfac;
+      });
   const float eps = S.xsph_eps;
   float vx = bi.x + dvx * eps, vy = bi.y + dvy * eps, vz = bi.z + dvz * eps;
   if (S.dim == 2) vz = 0.0f;
@@ -442,11 +485,19 @@ __global__ void k_vmax_zero(SceneState* __restrict__ st, uint32_t n_scenes) {
   if (k < n_scenes) st[k].vmax2_bits = 0u;
 }
 
-__global__ void k_vmax(uint32_t N, const float4* __restrict__ V, const uint32_t* __restrict__ META,
-                       SceneState* __restrict__ st) {
+__global__ void __launch_bounds__(256) k_vmax(uint32_t N, const float4* __restrict__ V,
+                                              const uint32_t* __restrict__ META, SceneState* __restrict__ st) {
   // No early return: every lane takes part in the shuffles.
+  __shared__ uint32_t blk_max;
+  __shared__ uint32_t blk_scene;
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x == 0) {
+    uint32_t s0 = blockIdx.x * blockDim.x;
+    blk_scene = (s0 < N) ? scene_of(META[s0]) : 0u;
+    blk_max = 0u;
+  }
+  __syncthreads();
   bool pending = s < N;
   uint32_t my_scene = 0;
   float v2 = 0.0f;
@@ -465,9 +516,16 @@ __global__ void k_vmax(uint32_t N, const float4* __restrict__ V, const uint32_t*
     uint32_t lscene = __shfl(my_scene, leader, kWave);
     float val = (pending && my_scene == lscene) ? v2 : 0.0f;
     for (int o = 32; o >= 1; o >>= 1) val = fmaxf(val, __shfl_xor(val, o, kWave));
-    if (lane == leader) atomicMax(&st[lscene].vmax2_bits, __float_as_uint(val));
+    if (lane == leader) {
+      if (lscene == blk_scene)
+        atomicMax(&blk_max, __float_as_uint(val));
+      else
+        atomicMax(&st[lscene].vmax2_bits, __float_as_uint(val));
+    }
     if (pending && my_scene == lscene) pending = false;
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x * blockDim.x < N) atomicMax(&st[blk_scene].vmax2_bits, blk_max);
 }
 
 __global__ void k_eos(uint32_t N, const SceneDev* __restrict__ sc, State cur, SceneState* __restrict__ st,
@@ -506,84 +564,64 @@ __global__ void k_eos(uint32_t N, const SceneDev* __restrict__ sc, State cur, Sc
 // ---------------------------------------------------------------------------
 __global__ void k_force(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
                         const SceneState* __restrict__ st, State cur, const uint32_t* __restrict__ LIST,
-                        const uint32_t* __restrict__ CNT, uint32_t cap, int integrate, int with_boundary) {
+                        const uint32_t* __restrict__ CNT, uint32_t cap, int integrate) {
   uint32_t G = gridDim.x;
   uint32_t blk = xcd_block(blockIdx.x, G);
   uint32_t s = blk * blockDim.x + threadIdx.x;
   if (s >= N) return;
   uint32_t meta = cur.META[s];
-  const SceneDev& S = sc[scene_of(meta)];
-  float4 ai = cur.A2[s];
-  float4 bi = cur.B2[s];
-  const float L = S.cell_f, r2s = S.r2_sup, inv_h = S.inv_h, aoh = S.alpha * S.inv_h;
-  uint32_t n = CNT[s];
-  const uint32_t* lp = LIST + ((uint64_t)(s >> 6) * cap) * kWave + (s & 63u);
   cur.IDF[s] = cur.ID[s];
   if (meta & kMetaBoundary) {
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (with_boundary) {
-      const float nm = -S.mass;
-      for (uint32_t k = 0; k < n; ++k) {
-        uint32_t e = lp[(uint64_t)k * kWave];
-        uint32_t j = e & kSlotMask;
-        float4 aj = cur.A2[j];
-        float ox, oy, oz;
-        entry_offset(e, L, ox, oy, oz);
-        // from the fluid particle's view: r = x_b - x_f
-        float rx = ai.x - (aj.x + ox), ry = ai.y - (aj.y + oy), rz = ai.z - (aj.z + oz);
-        float r2 = rx * rx + ry * ry + rz * rz;
-        if (!(r2 < r2s) || !(r2 > 0.0f)) continue;
-        float r = __builtin_amdgcn_sqrtf(r2);
-        float gf = -cubic_dwdr(r * inv_h, aoh) * __builtin_amdgcn_rcpf(r);
-        float Pj = cur.B2[j].w;
-        float sfac = gf * (ai.w * Pj) * nm;  // gw * (-(rho0 psi_b) P_f) * (-m)
-        fx += rx * sfac;
-        fy += ry * sfac;
-        fz += rz * sfac;
-      }
-    }
-    cur.BF[s] = make_float4(fx, fy, fz, 0.0f);
     cur.ACC[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
-  const float m = S.mass, h = S.h, va = S.visc_alpha, vb = S.visc_beta, eh2 = S.visc_eps_h2;
+  const SceneDev& S = sc[scene_of(meta)];
+  const float4 ai = cur.A2[s];
+  const float4 bi = cur.B2[s];
+  // Separations are in position units; q = r * q_unit, grad_i W = r_ij(units) * gf with
+  // gf = -dW/dr / r(units), and mu = (h/unit) v.r / (r^2 + eps h^2/unit^2), which equals
+  // the metre form exactly up to rounding.
+  const float r2s = S.r2s_u, qu = S.q_unit, aoh = S.alpha * S.inv_h;
+  const uint32_t n = CNT[s];
+  const uint4* lq = list_quads(LIST, s, cap);
+  const float m = S.mass, h = S.h_u, vb = S.visc_beta, eh2 = S.eh2_u;
   const float c = (float)st[scene_of(meta)].c;
+  const float vac = -S.visc_alpha * c;
   const float Pi = bi.w, rhoi = ai.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t e = lp[(uint64_t)k * kWave];
-    uint32_t j = e & kSlotMask;
-    float4 aj = cur.A2[j];
-    float ox, oy, oz;
-    entry_offset(e, L, ox, oy, oz);
-    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;  // x_j - x_i
-    float r2 = rx * rx + ry * ry + rz * rz;
-    if (!(r2 < r2s) || !(r2 > 0.0f)) continue;
-    float4 bj = cur.B2[j];
-    float r = __builtin_amdgcn_sqrtf(r2);
-    float gf = -cubic_dwdr(r * inv_h, aoh) * __builtin_amdgcn_rcpf(r);
-    float sfac;
-    if (!is_boundary_w(aj.w)) {
-      float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
-      float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
-      float pi = 0.0f;
-      if (vr < 0.0f) {
-        float mu = h * vr * __builtin_amdgcn_rcpf(r2 + eh2);
-        pi = (-va * c * mu + vb * mu * mu) * __builtin_amdgcn_rcpf(0.5f * (rhoi + aj.w));
-      }
-      sfac = gf * (-m * ((Pi + bj.w) + pi));
-    } else {
-      sfac = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
-    }
-    ax += rx * sfac;
-    ay += ry * sfac;
-    az += rz * sfac;
-  }
+  const float4* __restrict__ A2p = cur.A2;
+  const float4* __restrict__ B2p = cur.B2;
+  walk_list<Nb2>(
+      lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], B2p[j]}; },
+      [&](uint32_t e, const Nb2& nb) {
+        const float4 aj = nb.a, bj = nb.b;
+        (void)e;
+        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);  // x_j - x_i
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float rinv = __builtin_amdgcn_rsqf(r2);
+        float r = r2 * rinv;
+        float gf = -cubic_dwdr(r * qu, aoh) * rinv;  // grad_i W = (x_j - x_i) gf
+        // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
+        // with one reciprocal of d1 * rho_bar; zero for separating pairs.
+        float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+        float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
+        float d1 = r2 + eh2;
+        float d2 = 0.5f * (rhoi + aj.w);
+        float t = h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
+        float pi = t * (vac + vb * (t * d2));
+        pi = (vr < 0.0f) ? pi : 0.0f;
+        float sf_fluid = gf * (-m * ((Pi + bj.w) + pi));
+        float sf_bnd = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
+        float sfac = (aj.w > 0.0f) ? sf_fluid : sf_bnd;
+        sfac = (r2 < r2s && r2 > 0.0f) ? sfac : 0.0f;
+        ax += rx * sfac;
+        ay += ry * sfac;
+        az += rz * sfac;
+      });
   ax += S.g[0];
   ay += S.g[1];
   az += S.g[2];
   cur.ACC[s] = make_float4(ax, ay, az, 0.0f);
-  cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (!integrate) return;
   const double dt = ctl->dt;
   const float dtf = (float)dt;
@@ -595,20 +633,58 @@ __global__ void k_force(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, cons
   cur.X[s] = x;
   cur.Y[s] = y;
   cur.Z[s] = z;
-  uint32_t local = cur.CELL[s] - S.cell_base;
-  uint32_t lz = local % (uint32_t)S.nz;
-  uint32_t t = local / (uint32_t)S.nz;
-  uint32_t ly = t % (uint32_t)S.ny;
-  uint32_t lx = t / (uint32_t)S.ny;
-  double cx = (double)(S.ix0 + (int32_t)lx) * (double)S.cell_f;
-  double cy = (double)(S.iy0 + (int32_t)ly) * (double)S.cell_f;
-  double cz = (double)(S.iz0 + (int32_t)lz) * (double)S.cell_f;
-  cur.A[s] = make_float4((float)(x - cx), (float)(y - cy), (float)(z - cz), m);
+  cur.A[s] = make_float4(__int_as_float(to_units(x, S.org[0], S.inv_unit)),
+                         __int_as_float(to_units(y, S.org[1], S.inv_unit)),
+                         __int_as_float(to_units(z, S.org[2], S.inv_unit)), m);
   cur.B[s] = make_float4(vx, vy, vz, 0.0f);
   float d = cur.DISP[s] + sqrtf(vx * vx + vy * vy + vz * vz) * dtf;
   cur.DISP[s] = d;
   unsigned long long over = __ballot(!(d <= S.skin_half));  // also catches NaN; active lanes only
   if (over != 0ull && (int)(threadIdx.x & (kWave - 1)) == __ffsll((long long)over) - 1) ctlw->need_rebuild = 1u;
+}
+
+// Akinci reaction on boundary samples (wcsph.rs:253-254): each boundary
+// particle gathers -m f over the fluid particles that see it.
+__global__ void k_boundary_force(uint32_t N, const SceneDev* __restrict__ sc, State cur,
+                                 const uint32_t* __restrict__ LIST, const uint32_t* __restrict__ CNT, uint32_t cap,
+                                 int with_boundary) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  uint32_t meta = cur.META[s];
+  if (!(meta & kMetaBoundary)) {
+    cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (with_boundary) {
+    const SceneDev& S = sc[scene_of(meta)];
+    const float4 ai = cur.A2[s];
+    const float r2s = S.r2s_u, qu = S.q_unit, aoh = S.alpha * S.inv_h;
+    const float nm = -S.mass;
+    const uint32_t n = CNT[s];
+    const uint4* lq = list_quads(LIST, s, cap);
+    const float4* __restrict__ A2p = cur.A2;
+    const float4* __restrict__ B2p = cur.B2;
+    walk_list<Nb2>(
+        lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], B2p[j]}; },
+        [&](uint32_t e, const Nb2& nb) {
+          const float4 aj = nb.a;
+          const float Pj = nb.b.w;
+          (void)e;
+          // from the fluid particle's view: r = x_b - x_f
+          float rx = units_diff(ai.x, aj.x), ry = units_diff(ai.y, aj.y), rz = units_diff(ai.z, aj.z);
+          float r2 = rx * rx + ry * ry + rz * rz;
+          float rinv = __builtin_amdgcn_rsqf(r2);
+          float r = r2 * rinv;
+          float gf = -cubic_dwdr(r * qu, aoh) * rinv;
+          float sfac = gf * (ai.w * Pj) * nm;  // gw * (-(rho0 psi_b) P_f) * (-m)
+          sfac = (r2 < r2s && r2 > 0.0f) ? sfac : 0.0f;
+          fx += rx * sfac;
+          fy += ry * sfac;
+          fz += rz * sfac;
+        });
+  }
+  cur.BF[s] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -764,8 +840,11 @@ __global__ void k_stage(uint32_t N, const SceneDev* __restrict__ sc, State cur, 
   cur.B[i] = make_float4(vx, vy, vz, 0.0f);
   cur.B2[i] = make_float4(vx, vy, vz, 0.0f);
   float w = bnd ? -(float)(S.rho0_d * psi[i]) : S.mass;
-  cur.A[i] = make_float4(0.0f, 0.0f, 0.0f, w);
-  cur.A2[i] = make_float4(0.0f, 0.0f, 0.0f, bnd ? w : S.rho0);
+  const float ux = __int_as_float(to_units(x[3ull * i], S.org[0], S.inv_unit));
+  const float uy = __int_as_float(to_units(x[3ull * i + 1], S.org[1], S.inv_unit));
+  const float uz = __int_as_float(to_units(x[3ull * i + 2], S.org[2], S.inv_unit));
+  cur.A[i] = make_float4(ux, uy, uz, w);
+  cur.A2[i] = make_float4(ux, uy, uz, bnd ? w : S.rho0);
   cur.ID[i] = i;
   cur.IDF[i] = i;
   cur.META[i] = meta;
@@ -885,16 +964,13 @@ __global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, S
   const SceneDev& S = sc[scene_of(cur.META[slot])];
   float4 ai = cur.A[slot];
   uint32_t n = CNT[slot], m = 0;
-  const uint32_t* lp = LIST + ((uint64_t)(slot >> 6) * cap) * kWave + (slot & 63u);
   for (uint32_t k = 0; k < n; ++k) {
-    uint32_t e = lp[(uint64_t)k * kWave];
+    uint32_t e = LIST[list_index(slot, k, cap)];
     uint32_t j = e & kSlotMask;
     float4 aj = cur.A[j];
-    float ox, oy, oz;
-    entry_offset(e, S.cell_f, ox, oy, oz);
-    float rx = (aj.x + ox) - ai.x, ry = (aj.y + oy) - ai.y, rz = (aj.z + oz) - ai.z;
+    float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
     float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 < S.r2_sup) out[m++] = cur.ID[j];
+    if (r2 < S.r2s_u) out[m++] = cur.ID[j];
   }
   *out_len = m;
 }
@@ -911,6 +987,24 @@ __global__ void k_list_stats(uint32_t N, const uint32_t* __restrict__ CNT, Ctl* 
   uint32_t v = (s < N) ? CNT[s] : 0u;
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
   if ((threadIdx.x & 63) == 0) atomicAdd(&ctl->list_entries, (unsigned long long)v);
+}
+
+// The sentinel slot N: far outside every support radius, zero weight and
+// velocity; list padding points here.
+__global__ void k_sentinel(uint32_t N, State cur) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float far = __int_as_float(kSentinelUnits);
+  const float4 p = make_float4(far, far, far, 0.0f);
+  cur.A[N] = p;
+  cur.A2[N] = p;
+  cur.B[N] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cur.B2[N] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Plain streaming copy used to calibrate hardware byte counters.
+__global__ void k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 // ===========================================================================
@@ -980,8 +1074,8 @@ void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State
 void launch_force(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc, const SceneState* ss,
                   const State& cur, const uint32_t* LIST, const uint32_t* CNT, uint32_t cap, int integrate,
                   int with_boundary, uint32_t bs) {
-  PEF_LAUNCH(k_force, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, ss, cur, LIST, CNT, cap, integrate,
-             with_boundary);
+  PEF_LAUNCH(k_force, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, ss, cur, LIST, CNT, cap, integrate);
+  PEF_LAUNCH(k_boundary_force, blocks_for(N, 256), 256, st, N, sc, cur, LIST, CNT, cap, with_boundary);
 }
 
 void launch_diag(hipStream_t st, uint32_t first, uint32_t count, const SceneDev* sc, uint32_t scene, const State& cur,
@@ -1037,6 +1131,12 @@ void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, cons
 
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id) {
   PEF_LAUNCH(k_slot_of_id, blocks_for(N, 256), 256, st, N, cur, slot_of_id);
+}
+
+void launch_sentinel(hipStream_t st, uint32_t N, const State& cur) { PEF_LAUNCH(k_sentinel, 1, 64, st, N, cur); }
+
+void launch_calib_copy(hipStream_t st, const float4* src, float4* dst, uint64_t n) {
+  PEF_LAUNCH(k_calib_copy, blocks_for(n, 256), 256, st, src, dst, n);
 }
 
 void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl) {

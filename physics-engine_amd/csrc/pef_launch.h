@@ -66,6 +66,8 @@ void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, cons
                          const uint32_t* CNT, uint32_t cap, uint64_t* out, uint32_t* out_len);
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id);
 void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl);
+void launch_calib_copy(hipStream_t st, const float4* src, float4* dst, uint64_t n);
+void launch_sentinel(hipStream_t st, uint32_t N, const State& cur);
 
 // pef_setup.hip
 int akinci_psi_gpu(hipStream_t st, int dim, double h, uint64_t nb, const double* bx_host, double* psi_host);

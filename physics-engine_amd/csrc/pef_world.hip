@@ -144,6 +144,16 @@ int32_t check_params(const PefParams& p) {
   return PEF_OK;
 }
 
+// Unit-scaled constants of the integer position frame (S.unit_d must be set).
+void fill_units(pef::SceneDev& S, const PefParams& p, double skin) {
+  const double sup = 2.0 * p.h, rl = sup * (1.0 + skin), u = S.unit_d;
+  S.q_unit = (float)(u / p.h);
+  S.r2s_u = (float)((sup / u) * (sup / u));
+  S.r2l_u = (float)((rl / u) * (rl / u));
+  S.h_u = (float)(p.h / u);
+  S.eh2_u = (float)(p.visc_eps * p.h * p.h / (u * u));
+}
+
 // Per-scene device constants from params and the grid placement.
 void fill_scene_physics(pef::SceneDev& S, const PefParams& p, double skin) {
   S.dim = p.dim;
@@ -216,9 +226,21 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     if (opt->use_graph < 0) w.use_graph = 0;
     if (opt->block_size) w.bs = opt->block_size;
   }
-  if (w.cap < 8 || w.cap > 4096 || !(w.skin >= 0.0 && w.skin <= 1.0) || w.bs % 64 != 0 || w.bs > 1024 ||
-      w.margin > 1024)
-    return fail(PEF_ERR_ARG);
+  if (!(w.skin >= 0.0 && w.skin <= 1.0) || w.bs % 64 != 0 || w.bs > 1024 || w.margin > 1024) return fail(PEF_ERR_ARG);
+  if (!(opt && opt->neighbour_capacity)) {
+    // Default capacity: 1.3x the particles a rest-density lattice puts inside
+    // the list radius 2h (1 + skin), plus 32, rounded up to a multiple of 32.
+    double need = 0.0;
+    for (uint32_t k = 0; k < n_scenes; ++k) {
+      const PefParams& p = params[k];
+      double r = 2.0 * p.h * (1.0 + w.skin), n = p.rest_density / p.mass;
+      double cnt = (p.dim == 3) ? 4.18879020478639 * r * r * r * n : 3.14159265358979 * r * r * n;
+      need = std::max(need, 1.3 * cnt + 32.0);
+    }
+    w.cap = (uint32_t)std::min(4096.0, std::ceil(need / 32.0) * 32.0);
+  }
+  w.cap = (w.cap + 3u) & ~3u;  // the list is read four entries at a time
+  if (w.cap < 8 || w.cap > 4096) return fail(PEF_ERR_ARG);
   w.S = n_scenes;
   w.N = (uint32_t)Ntot;
   w.params.assign(params, params + n_scenes);
@@ -266,6 +288,13 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     S.nx = (int32_t)dims[0];
     S.ny = (int32_t)dims[1];
     S.nz = (int32_t)dims[2];
+    // integer position frame: origin at the grid centre, 2^30 units span the grid
+    const double ext = (double)std::max(dims[0], std::max(dims[1], dims[2])) * S.cell;
+    S.unit_d = ext / 1073741824.0;
+    S.inv_unit = 1.0 / S.unit_d;
+    for (int a = 0; a < 3; ++a) S.org[a] = ((double)lo[a] + 0.5 * (double)dims[a]) * S.cell;
+    if (p.dim == 2) S.org[2] = 0.0;
+    fill_units(S, p, w.skin);
     S.cell_base = (uint32_t)cells;
     S.slot_first = (uint32_t)first;
     S.slot_count = (uint32_t)P.n;
@@ -294,10 +323,10 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     ALLOC(s.X, N);
     ALLOC(s.Y, N);
     ALLOC(s.Z, N);
-    ALLOC(s.A, N);
-    ALLOC(s.A2, N);
-    ALLOC(s.B, N);
-    ALLOC(s.B2, N);
+    ALLOC(s.A, (size_t)N + 1);  // + sentinel slot
+    ALLOC(s.A2, (size_t)N + 1);  // + sentinel slot
+    ALLOC(s.B, (size_t)N + 1);  // + sentinel slot
+    ALLOC(s.B2, (size_t)N + 1);  // + sentinel slot
     ALLOC(s.ID, N);
     ALLOC(s.META, N);
     ALLOC(s.CELL, N);
@@ -375,6 +404,7 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   if (e == hipSuccess) e = hipMemsetAsync(w.CNT, 0, (size_t)N * 4, st);
   if (e == hipSuccess) {
     pef::launch_stage(st, N, w.sc_d, w.cur, dx, dv, dk, dpsi, dn, dscene);
+    pef::launch_sentinel(st, N, w.cur);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -996,6 +1026,7 @@ int32_t pef_world_set_params(PefWorld* world, uint32_t scene, const PefParams* p
   return guard([&] {
     w.params[scene] = *params;
     fill_scene_physics(w.scenes[scene], *params, w.skin);
+    fill_units(w.scenes[scene], *params, w.skin);
     HIPTRY(hipMemcpyAsync(w.sc_d + scene, &w.scenes[scene], sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
     pef::SceneState s;
     s.c = params->sound_speed;
@@ -1086,3 +1117,57 @@ int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* re
 }
 
 }  // extern "C"
+
+// Internal debugging aid (not part of the public ABI): copies one raw device
+// array of the world into `out` (bytes). which: 0 A, 1 A2, 2 B, 3 B2, 4 ID,
+// 5 META, 6 CELL, 7 CNT, 8 LIST, 9 cell_start, 10 X, 11 Y, 12 Z.
+extern "C" int32_t pefx_debug_dump(PefWorld* world, int32_t which, void* out, uint64_t bytes, uint64_t* need) {
+  if (!world || !need) return PEF_ERR_NULL;
+  World& w = world->w;
+  const void* src = nullptr;
+  uint64_t n = 0;
+  const uint64_t N = w.N;
+  switch (which) {
+    case 0: src = w.cur.A; n = N * 16; break;
+    case 1: src = w.cur.A2; n = N * 16; break;
+    case 2: src = w.cur.B; n = N * 16; break;
+    case 3: src = w.cur.B2; n = N * 16; break;
+    case 4: src = w.cur.ID; n = N * 4; break;
+    case 5: src = w.cur.META; n = N * 4; break;
+    case 6: src = w.cur.CELL; n = N * 4; break;
+    case 7: src = w.CNT; n = N * 4; break;
+    case 8: src = w.LIST; n = ((N + 63) / 64) * 64ull * w.cap * 4; break;
+    case 9: src = w.cell_start; n = ((uint64_t)w.C + 1) * 4; break;
+    case 10: src = w.cur.X; n = N * 8; break;
+    case 11: src = w.cur.Y; n = N * 8; break;
+    case 12: src = w.cur.Z; n = N * 8; break;
+    default: return PEF_ERR_ARG;
+  }
+  *need = n;
+  if (!out || bytes < n) return PEF_ERR_BOUNDS;
+  if (hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  if (hipStreamSynchronize(w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  return PEF_OK;
+}
+
+// Internal measurement aid: one launch of a plain copy kernel moving `bytes`
+// (read) + `bytes` (write), for calibrating hardware byte counters.
+extern "C" int32_t pefx_calibration_copy(uint64_t bytes) {
+  uint64_t n = bytes / 16;
+  if (n == 0) return PEF_ERR_ARG;
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess) return PEF_ERR_PANIC;
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return PEF_ERR_PANIC;
+  }
+  hipError_t e = hipMemset(a, 1, n * 16);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) {
+    pef::launch_calib_copy(nullptr, a, b, n);
+    e = hipDeviceSynchronize();
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return e == hipSuccess ? PEF_OK : PEF_ERR_PANIC;
+}

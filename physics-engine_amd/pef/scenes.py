@@ -6,9 +6,17 @@ Every parameter the configurations leave open is fixed here once:
     (fill_box_boundary + recomputed_akinci_psi), the reference's only WCSPH
     wall model (wcsph.rs:144-149, 244-255);
   - particle mass = mass_calibrated_to_rest(rho0, dx, h) (particles.rs:178-215);
-  - unstated WCSPH parameters keep WcsphParams::water defaults
-    (XSPH 0.25, adaptive sound speed, beta 0, eps 0.01);
-  - the fluid block starts at (dx/2, dx/2, dx/2) in the tank corner;
+  - the fluid block starts 2 dx from the walls, at (2 dx, 2 dx, 2 dx) in the
+    tank corner, as in the reference's 3-D WCSPH bench (benches/fluid.rs:65-96:
+    block at 0.1 m, walls at 0, dx 0.05); closer blocks start with wall
+    densities of 1.4-1.6 rho0 under a one-layer Akinci wall;
+  - the sound speed is fixed (adapt_sound_speed = false), as in every WCSPH
+    dam-break and validation run of the reference (validation.rs:430,
+    tests/coupling_oracles.rs:73, tests/fluid_analytic.rs:122, 282): with
+    c >= 10 v_max the free-surface start-up transient feeds back through
+    B = rho0 c^2 / gamma and the f64 step itself diverges within 60 steps;
+  - other unstated WCSPH parameters keep WcsphParams::water defaults
+    (XSPH 0.25, beta 0, eps 0.01);
   - jitter: per fluid particle in fill order, per component x, y, z,
     uniform(-0.01 dx, 0.01 dx) from xoshiro256** seeded with `seed`.
 """
@@ -38,6 +46,8 @@ class DamBreak:
     visc_alpha: float = 0.05
     gravity: Tuple[float, float, float] = (0.0, 0.0, -9.81)
     dim: int = 3
+    wall_gap: float = 2.0  # distance of the first fluid layer from the walls, in dx
+    adapt_sound_speed: bool = False
 
     @property
     def h(self) -> float:
@@ -48,21 +58,22 @@ class DamBreak:
         return self.nx * self.ny * self.nz
 
 
+# The block spans the tank width in y: tank y = (ny - 1 + 2 * wall_gap) dx.
 # 8,000-particle CPU case: 20^3 block, dx 0.02, 1.0 x 0.5 x 0.5 m tank, 1,000 steps at 1e-4 s.
 CPU_8K = DamBreak("dam_break_8k", 20, 20, 20, 0.02, (1.0, 0.5, 0.5), 1e-4, 1000, jitter=0.0)
 # 65,536-particle parity scene: 64 x 32 x 32 at dx 0.005, 200 steps at 2.5e-5 s.
-CHECK_64K = DamBreak("dam_break_64k", 64, 32, 32, 0.005, (0.8, 0.16, 0.2), 2.5e-5, 200)
+CHECK_64K = DamBreak("dam_break_64k", 64, 32, 32, 0.005, (0.8, 0.175, 0.2), 2.5e-5, 200)
 # 1,048,576-particle GPU benchmark: 128 x 64 x 128 at dx 0.005, 500 steps at 2.5e-5 s.
-GPU_1M = DamBreak("dam_break_1m", 128, 64, 128, 0.005, (1.6, 0.32, 0.8), 2.5e-5, 500)
+GPU_1M = DamBreak("dam_break_1m", 128, 64, 128, 0.005, (1.6, 0.335, 0.8), 2.5e-5, 500)
 # 8,388,608-particle large run: 256 x 128 x 256 at dx 0.005.
-GPU_8M = DamBreak("dam_break_8m", 256, 128, 256, 0.005, (3.2, 0.64, 1.6), 2.5e-5, 2000)
+GPU_8M = DamBreak("dam_break_8m", 256, 128, 256, 0.005, (3.2, 0.655, 1.6), 2.5e-5, 2000)
 
 CONFIGS = {c.name: c for c in (CPU_8K, CHECK_64K, GPU_1M, GPU_8M)}
 
 
 def small(nx=12, ny=8, nz=10, dx=0.01, steps=20, jitter=0.01, dim=3) -> DamBreak:
     """A small dam break for quick parity runs (tank 2.5x the block in x)."""
-    tank = (round(2.5 * nx * dx, 10), round(ny * dx + 2 * dx, 10), round(1.25 * nz * dx + 2 * dx, 10))
+    tank = (round(2.5 * nx * dx, 10), round((ny + 3) * dx, 10), round(1.25 * nz * dx + 3 * dx, 10))
     if dim == 2:
         tank = (tank[0], tank[2], 0.0)
     return DamBreak(f"small_{nx}x{ny}x{nz}", nx, ny, nz, dx, tank, 2.5e-5 * dx / 0.005, steps, jitter=jitter,
@@ -75,7 +86,7 @@ def build(cfg: DamBreak):
 
     m = pef.mass_calibrated_to_rest(cfg.rest_density, cfg.dx, cfg.h, cfg.dim)
     parts = pef.Particles(cfg.dim, cfg.rest_density, m, cfg.h)
-    o = 0.5 * cfg.dx
+    o = cfg.wall_gap * cfg.dx
     origin = (o, o, o) if cfg.dim == 3 else (o, o, 0.0)
     pef.fill_box(parts, origin, cfg.nx, cfg.ny, cfg.nz, cfg.dx)
     if cfg.jitter:
@@ -84,4 +95,5 @@ def build(cfg: DamBreak):
     pef.recomputed_akinci_psi(parts)
     params = pef.WcsphParams.water(cfg.rest_density, cfg.sound_speed, cfg.gravity)
     params.visc_alpha = cfg.visc_alpha
+    params.adapt_sound_speed = cfg.adapt_sound_speed
     return parts, params

@@ -12,7 +12,7 @@ def oracle_scene(cfg):
     """(params dict, arrays dict) of the oracle for a DamBreak config."""
     m = O.mass_calibrated_to_rest(cfg.rest_density, cfg.dx, cfg.h, cfg.dim)
     P = O.Particles(cfg.dim, cfg.rest_density, m, cfg.h)
-    o = 0.5 * cfg.dx
+    o = cfg.wall_gap * cfg.dx
     P.fill_box((o, o, o) if cfg.dim == 3 else (o, o, 0.0), cfg.nx, cfg.ny, cfg.nz, cfg.dx)
     if cfg.jitter:
         P.jitter(cfg.seed, cfg.jitter * cfg.dx)
@@ -20,7 +20,7 @@ def oracle_scene(cfg):
     P.recompute_psi()
     a = P.arrays()
     prm = dict(dim=cfg.dim, rest_density=cfg.rest_density, mass=m, h=cfg.h, sound_speed=cfg.sound_speed,
-               visc_alpha=cfg.visc_alpha, gravity=cfg.gravity)
+               visc_alpha=cfg.visc_alpha, gravity=cfg.gravity, adapt_sound_speed=cfg.adapt_sound_speed)
     return prm, a
 
 

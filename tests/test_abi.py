@@ -100,7 +100,7 @@ def test_scene_configs_particle_counts():
     def shell(n):
         nx, ny, nz = n
         return 2 * nx * ny + 2 * nx * (nz - 2) + 2 * (ny - 2) * (nz - 2)
-    for cfg, nb in [(scenes.CPU_8K, 6252), (scenes.CHECK_64K, 25602), (scenes.GPU_1M, 163842)]:
+    for cfg, nb in [(scenes.CPU_8K, 6252), (scenes.CHECK_64K, 26802), (scenes.GPU_1M, 166722)]:
         dims = tuple(int(round(L / cfg.dx)) + 1 for L in cfg.tank)
         assert shell(dims) == nb
     assert scenes.GPU_1M.n_fluid == 1048576 and scenes.CHECK_64K.n_fluid == 65536
