@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--lpp", type=int, default=0, help="lanes per particle of the pair kernels (0 = default)")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size of the pair kernels (0 = default)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this size (counter calibration)")
     args = ap.parse_args()
@@ -86,7 +88,7 @@ def main():
     cfg = scenes.CONFIGS[args.config]
     t_setup = time.perf_counter()
     parts, params = scenes.build(cfg)
-    W = pef.WcsphWorld(parts, params, pef.Options(skin=args.skin))
+    W = pef.WcsphWorld(parts, params, pef.Options(skin=args.skin, lanes_per_particle=args.lpp, block_size=args.block))
     n_fluid = parts.fluid_count()
     n_bnd = len(parts) - n_fluid
     t_setup = time.perf_counter() - t_setup
@@ -138,6 +140,7 @@ def main():
                    "surrogate_speedup": None},
         "roofline": roof,
         "kernel_ms": {k: round(v, 5) for k, v in per_launch_ms.items()},
+        "tiles": W.tile_info(),
         "mean_density_error": diag["mean_density_error"],
     }
     if args.calibration_copy_mib:

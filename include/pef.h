@@ -100,6 +100,11 @@ typedef struct PefOptions {
   int32_t boundary_force;      /* compute the Akinci reactions each step (default 1; -1 = off) */
   int32_t use_graph;           /* replay captured HIP graphs for multi-step calls (default 1; -1 = off) */
   uint32_t block_size;         /* threads per workgroup of the pair kernels (0 = tuned default) */
+  uint32_t lanes_per_particle; /* lanes sharing one particle's list in the pair kernels: 2, 4, 8 or
+                                  16 (0 = tuned default) */
+  uint32_t lds_halo_capacity;  /* upper bound on the halo particles a pair kernel stages in LDS
+                                  per brick (0 = as many as LDS holds); larger halos read their
+                                  partners from global memory, with identical results */
 } PefOptions;
 
 /* Global conservation and error diagnostics, reduced in FP64 on the device

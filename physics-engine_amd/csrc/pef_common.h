@@ -15,10 +15,13 @@
 //   META      u32  bit0 boundary, bit1 oriented boundary, bits 8.. scene index
 //   CELL      u32  global id of the cell at the last list build
 //   DISP      f32  displacement bound since the last list build
-//   LIST      u32  neighbour slots, blocked ELL by 64-slot chunk:
-//             LIST[((chunk * cap/4 + k/4) * 64 + lane) * 4 + k%4]; each list is
-//             in ascending slot order and padded to a multiple of 4 with the
-//             sentinel slot N
+//   LIST      u16  neighbours as halo-local indices (see pef_pair.hip),
+//             contiguous per slot: LIST[s * cap + k]; each list is in
+//             ascending halo-local order and padded to a multiple of 4 with
+//             kListPad
+//
+// Internal cells are numbered brick-major: bricks of 2 x 2 x 1 cells, the four
+// cells of a brick consecutive, so a brick's particles form one slot range.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +42,9 @@ constexpr uint32_t kMetaSceneShift = 8u;
 constexpr uint32_t kFlagGrid = 1u;
 constexpr uint32_t kFlagNeighbours = 2u;
 
+// List padding entry (never a halo-local index).
+constexpr uint32_t kListPad = 0xFFFFu;
+
 // Per-scene constants (device copy).
 struct SceneDev {
   // internal (list) grid: cell = 2h (1 + skin)
@@ -53,6 +59,8 @@ struct SceneDev {
   int32_t adapt;
   int32_t gamma_int;  // gamma when it is an integer in [1, 16], else 0
   int32_t oriented;   // scene holds one-sided boundary samples
+  int32_t bricked;    // 1: brick-major cell ids (list grid); 0: (ix, iy, iz) lexicographic (reference grid)
+  int32_t nbx, nby;   // bricks along x and y (bricked grids; bricks along z = nz)
   float cell_f;
   float h, inv_h, mass, rho0, inv_rho0;
   float alpha;       // cubic spline normalisation
@@ -92,7 +100,7 @@ struct Ctl {
   uint32_t rebuilding;     // latched for the current step by k_decide
   uint32_t flags;          // sticky kFlag* bits
   uint32_t max_count;      // largest list length seen at the last build
-  uint32_t pad;
+  uint32_t slow_bricks;    // bricks whose halo exceeded the LDS capacity at the last build
   unsigned long long list_entries;  // total list entries at the last build
   unsigned long long rebuilds;
 };
@@ -135,6 +143,14 @@ __device__ __host__ inline int32_t to_units(double x, double org, double inv_uni
 }
 __device__ inline float units_diff(float a, float b) {
   return (float)(__float_as_int(a) - __float_as_int(b));  // exact: |difference| < 2^24 for neighbours
+}
+
+// Cell id of in-grid cell coordinates (relative to the scene's grid corner).
+constexpr uint32_t kBrickCells = 4;  // 2 x 2 x 1
+__device__ __host__ inline uint32_t cell_id(const SceneDev& S, uint32_t lx, uint32_t ly, uint32_t lz) {
+  if (!S.bricked) return S.cell_base + (lx * (uint32_t)S.ny + ly) * (uint32_t)S.nz + lz;
+  const uint32_t b = ((lx >> 1) * (uint32_t)S.nby + (ly >> 1)) * (uint32_t)S.nz + lz;
+  return S.cell_base + b * kBrickCells + ((lx & 1u) << 1) + (ly & 1u);
 }
 
 __device__ inline uint32_t scene_of(uint32_t meta) { return meta >> kMetaSceneShift; }

@@ -3,12 +3,10 @@
 // Kernel map (reference function -> kernel):
 //   UniformGrid::key_of + rebuild (grid.rs:62-107)  -> k_cell_count, k_scan_*,
 //                                                      k_scatter, k_finalize
-//   UniformGrid::neighbours (grid.rs:122-140)        -> k_nlist (filtered list)
-//   WcsphWorld::compute_density (wcsph.rs:117-154)   -> k_density
 //   compute_pressure (wcsph.rs:157-178)              -> k_vmax + k_eos
-//   compute_forces + kick/drift (wcsph.rs:181-342)   -> k_force
-//   apply_xsph (wcsph.rs:283-323)                    -> k_xsph
 //   diagnostics (diagnostics.rs:19-102)              -> k_diag_partial/final
+// The neighbour lists and the pair kernels (density, XSPH, forces) live in
+// pef_pair.hip.
 #include "pef_common.h"
 #include "pef_launch.h"
 
@@ -28,7 +26,10 @@ __global__ void k_decide(Ctl* ctl, SceneState* st, uint32_t n_scenes) {
     ctl->rebuilding = r;
     ctl->need_rebuild = 0u;
     ctl->force_rebuild = 0u;
-    if (r) ctl->rebuilds += 1ull;
+    if (r) {
+      ctl->rebuilds += 1ull;
+      ctl->slow_bricks = 0u;
+    }
   }
   (void)st;
   (void)n_scenes;
@@ -57,7 +58,7 @@ __global__ void k_cell_count(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw,
     ly = ly < 0 ? 0 : (ly >= S.ny ? S.ny - 1 : ly);
     lz = lz < 0 ? 0 : (lz >= S.nz ? S.nz - 1 : lz);
   }
-  uint32_t cid = S.cell_base + ((uint32_t)lx * (uint32_t)S.ny + (uint32_t)ly) * (uint32_t)S.nz + (uint32_t)lz;
+  uint32_t cid = cell_id(S, (uint32_t)lx, (uint32_t)ly, (uint32_t)lz);
   cell_of[s] = cid;
   rank_of[s] = atomicAdd(&counts[cid], 1u);
 }
@@ -238,245 +239,6 @@ __global__ void k_copyback(uint32_t N, const Ctl* __restrict__ ctl, State alt, S
   }
 }
 
-// Blocked ELL list layout: the entries of a 64-slot chunk are stored four per
-// lane, LIST[((chunk * cap/4 + k/4) * 64 + lane) * 4 + k%4], so one 16-byte load
-// gives a lane four consecutive entries and a wave reads 1 KiB contiguous.
-__device__ inline uint64_t list_index(uint32_t s, uint32_t k, uint32_t cap) {
-  return ((((uint64_t)(s >> 6) * (cap >> 2) + (k >> 2)) * kWave + (s & 63u)) << 2) + (k & 3u);
-}
-__device__ inline const uint4* list_quads(const uint32_t* LIST, uint32_t s, uint32_t cap) {
-  return reinterpret_cast<const uint4*>(LIST) + (uint64_t)(s >> 6) * (cap >> 2) * kWave + (s & 63u);
-}
-// The list is streamed once per pass: load it non-temporally so it does not
-// evict the neighbour state the gathers reuse from L2.
-typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-__device__ inline uint4 load_quad(const uint4* lq, uint32_t k) {
-  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(lq + (uint64_t)(k >> 2) * kWave));
-  return make_uint4(v.x, v.y, v.z, v.w);
-}
-
-// Software-pipelined walk over a particle's list: while quad k (four entries)
-// is evaluated, the neighbour gathers of quad k+1 and the list load of quad
-// k+2 are already in flight. Lists are padded to a multiple of four with the
-// sentinel slot (index N: a particle far outside every support radius), so
-// every entry of a quad is a valid gather and contributes nothing when it is
-// padding. `gather(j)` loads the neighbour state of slot j;
-// `body(entry, state)` evaluates one pair.
-template <class T, class Gather, class Body>
-__device__ inline void walk_list(const uint4* lq, uint32_t n, uint32_t sentinel, Gather gather, Body body) {
-  if (n == 0) return;
-  const uint4 pad = make_uint4(sentinel, sentinel, sentinel, sentinel);
-  uint4 e0 = load_quad(lq, 0);
-  uint4 e1 = (n > 4) ? load_quad(lq, 4) : pad;
-  T g0 = gather(e0.x & kSlotMask), g1 = gather(e0.y & kSlotMask), g2 = gather(e0.z & kSlotMask),
-    g3 = gather(e0.w & kSlotMask);
-  for (uint32_t k = 0; k < n; k += 4) {
-    T h0 = gather(e1.x & kSlotMask), h1 = gather(e1.y & kSlotMask), h2 = gather(e1.z & kSlotMask),
-      h3 = gather(e1.w & kSlotMask);
-    const uint4 e2 = (k + 8 < n) ? load_quad(lq, k + 8) : pad;
-    body(e0.x, g0);
-    body(e0.y, g1);
-    body(e0.z, g2);
-    body(e0.w, g3);
-    e0 = e1;
-    e1 = e2;
-    g0 = h0;
-    g1 = h1;
-    g2 = h2;
-    g3 = h3;
-  }
-}
-
-struct Nb2 {
-  float4 a, b;
-};
-
-// ---------------------------------------------------------------------------
-// neighbour-list build (grid.rs:122-140 candidates, filtered to the list radius)
-// ---------------------------------------------------------------------------
-// One-sided Akinci filter (particles.rs:126-134) in FP64, evaluated like the
-// reference with no contraction.
-__device__ inline bool sees_f64(double bx, double by, double bz, double nx, double ny, double nz, double fx,
-                                double fy, double fz, double h) {
-#pragma clang fp contract(off)
-  double n2 = nx * nx + ny * ny + nz * nz;
-  if (n2 <= 0.0) return true;
-  double eps = 0.15 * h * sqrt(n2 > 0.0 ? n2 : 0.0);
-  double d = (fx - bx) * nx + (fy - by) * ny + (fz - bz) * nz;
-  return d > -eps;
-}
-
-__global__ void k_nlist(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
-                        const uint32_t* __restrict__ cell_start, State cur, uint32_t* __restrict__ LIST,
-                        uint32_t* __restrict__ CNT, uint32_t cap, int with_boundary, int always) {
-  if (!always && !ctl->rebuilding) return;  // uniform across the grid
-  uint32_t G = gridDim.x;
-  uint32_t blk = xcd_block(blockIdx.x, G);
-  uint32_t s = blk * blockDim.x + threadIdx.x;
-  uint32_t cnt = 0;
-  if (s < N) {
-    uint32_t meta = cur.META[s];
-    const SceneDev& S = sc[scene_of(meta)];
-    const bool i_bnd = (meta & kMetaBoundary) != 0u;
-    if (!i_bnd || with_boundary) {
-      float4 ai = cur.A[s];
-      uint32_t c = cur.CELL[s];
-      uint32_t local = c - S.cell_base;
-      int32_t lz = (int32_t)(local % (uint32_t)S.nz);
-      uint32_t t = local / (uint32_t)S.nz;
-      int32_t ly = (int32_t)(t % (uint32_t)S.ny);
-      int32_t lx = (int32_t)(t / (uint32_t)S.ny);
-      const float r2l = S.r2l_u;
-      const bool i_or = (meta & kMetaOriented) != 0u;
-      // Stencil cells in ascending cell id (dx outer, dz inner), so each list
-      // is in ascending slot order and the k-th entries of neighbouring lanes
-      // stay close in memory when the pair kernels gather them.
-      const int zlo = (S.dim == 2) ? 0 : -1, zhi = (S.dim == 2) ? 0 : 1;
-      for (int dx = -1; dx <= 1; ++dx) {
-        int32_t x = lx + dx;
-        if (x < 0 || x >= S.nx) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-          int32_t y = ly + dy;
-          if (y < 0 || y >= S.ny) continue;
-          for (int dz = zlo; dz <= zhi; ++dz) {
-            int32_t z = lz + dz;
-            if (z < 0 || z >= S.nz) continue;
-            uint32_t cc = S.cell_base + ((uint32_t)x * (uint32_t)S.ny + (uint32_t)y) * (uint32_t)S.nz + (uint32_t)z;
-            uint32_t b = cell_start[cc], e = cell_start[cc + 1];
-            for (uint32_t j = b; j < e; ++j) {
-              float4 aj = cur.A[j];
-              float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
-              float r2 = rx * rx + ry * ry + rz * rz;
-              if (!(r2 < r2l) || j == s) continue;
-              const bool j_bnd = is_boundary_w(aj.w);
-              bool ok;
-              if (!i_bnd) {
-                ok = true;
-                if (j_bnd && S.oriented && (cur.META[j] & kMetaOriented))
-                  ok = sees_f64(cur.X[j], cur.Y[j], cur.Z[j], cur.NX[j], cur.NY[j], cur.NZ[j], cur.X[s], cur.Y[s],
-                                cur.Z[s], S.h_d);
-              } else {
-                ok = !j_bnd;
-                if (ok && i_or)
-                  ok = sees_f64(cur.X[s], cur.Y[s], cur.Z[s], cur.NX[s], cur.NY[s], cur.NZ[s], cur.X[j], cur.Y[j],
-                                cur.Z[j], S.h_d);
-              }
-              if (!ok) continue;
-              if (cnt < cap) LIST[list_index(s, cnt, cap)] = j;
-              ++cnt;
-            }
-          }
-        }
-      }
-    }
-    const uint32_t stored = cnt < cap ? cnt : cap;
-    for (uint32_t k = stored; k < ((stored + 3u) & ~3u); ++k) LIST[list_index(s, k, cap)] = N;  // sentinel
-    CNT[s] = stored;
-  }
-  // wave max of the counts (all lanes present), one atomic per wave
-  uint32_t mx = cnt;
-  for (int o = 32; o >= 1; o >>= 1) {
-    uint32_t t2 = __shfl_xor(mx, o, kWave);
-    mx = mx > t2 ? mx : t2;
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicMax(&ctlw->max_count, mx);
-    if (mx > cap) atomicOr(&ctlw->flags, kFlagNeighbours);
-  }
-}
-
-
-// ---------------------------------------------------------------------------
-// density summation with a compensated FP32 sum (wcsph.rs:117-154)
-// ---------------------------------------------------------------------------
-__global__ void k_density(uint32_t N, const SceneDev* __restrict__ sc, State cur, const uint32_t* __restrict__ LIST,
-                          const uint32_t* __restrict__ CNT, uint32_t cap) {
-  uint32_t G = gridDim.x;
-  uint32_t blk = xcd_block(blockIdx.x, G);
-  uint32_t s = blk * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  float4 ai = cur.A[s];
-  if (is_boundary_w(ai.w)) {
-    cur.A2[s] = ai;  // boundary keeps w = -rho0 psi; its density is rho0 (wcsph.rs:126-129)
-    return;
-  }
-  const SceneDev& S = sc[scene_of(cur.META[s])];
-  const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha;
-  // sum of weight * W / alpha (alpha is applied once at the end); self term m W(0)
-  float sum = S.mass;
-  float comp = 0.0f;
-  const uint32_t n = CNT[s];
-  const uint4* lq = list_quads(LIST, s, cap);
-  const float4* __restrict__ Ap = cur.A;
-  walk_list<float4>(
-      lq, n, N, [&](uint32_t j) { return Ap[j]; },
-      [&](uint32_t e, const float4& aj) {
-        (void)e;
-        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
-        float r2 = rx * rx + ry * ry + rz * rz;
-        float r = __builtin_amdgcn_sqrtf(r2);
-        float w = fabsf(aj.w) * cubic_w(r * qu, 1.0f);
-        float term = (r2 < r2s) ? w : 0.0f;
-        float y = term - comp;  // compensated (Kahan) sum
-        float tt = sum + y;
-        comp = (tt - sum) - y;
-        sum = tt;
-      });
-  sum *= alpha;
-  float rho = fmaxf(sum, 1e-12f);
-  cur.A2[s] = make_float4(ai.x, ai.y, ai.z, rho);
-}
-
-// ---------------------------------------------------------------------------
-// XSPH velocity smoothing (wcsph.rs:283-323), Jacobi: reads B, writes B2
-// ---------------------------------------------------------------------------
-__global__ void k_xsph(uint32_t N, const SceneDev* __restrict__ sc, State cur, const uint32_t* __restrict__ LIST,
-                       const uint32_t* __restrict__ CNT, uint32_t cap) {
-  uint32_t G = gridDim.x;
-  uint32_t blk = xcd_block(blockIdx.x, G);
-  uint32_t s = blk * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  float4 ai = cur.A2[s];
-  float4 bi = cur.B[s];
-  const float Pold = cur.B2[s].w;
-  if (is_boundary_w(ai.w)) {
-    cur.B2[s] = make_float4(bi.x, bi.y, bi.z, Pold);
-    return;
-  }
-  const SceneDev& S = sc[scene_of(cur.META[s])];
-  if (S.xsph_eps == 0.0f) {
-    cur.B2[s] = make_float4(bi.x, bi.y, bi.z, Pold);
-    return;
-  }
-  const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha, m = S.mass;
-  float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
-  const uint32_t n = CNT[s];
-  const uint4* lq = list_quads(LIST, s, cap);
-  const float4* __restrict__ A2p = cur.A2;
-  const float4* __restrict__ Bp = cur.B;
-  const float two_m = 2.0f * m;
-  walk_list<Nb2>(
-      lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], Bp[j]}; },
-      [&](uint32_t e, const Nb2& nb) {
-        const float4 aj = nb.a, bj = nb.b;
-        (void)e;
-        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
-        float r2 = rx * rx + ry * ry + rz * rz;
-        float r = __builtin_amdgcn_sqrtf(r2);
-        float wij = cubic_w(r * qu, alpha);
-        // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2
-        float sfac = two_m * wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
-        sfac = (r2 < r2s && aj.w > 0.0f) ? sfac : 0.0f;  // fluid neighbours only
-        dvx += (bj.x - bi.x) * sfac;
-        dvy += (bj.y - bi.y) * sfac;
-        dvz += (bj.z - bi.z) * sfac;
-      });
-  const float eps = S.xsph_eps;
-  float vx = bi.x + dvx * eps, vy = bi.y + dvy * eps, vz = bi.z + dvz * eps;
-  if (S.dim == 2) vz = 0.0f;
-  cur.B2[s] = make_float4(vx, vy, vz, Pold);
-}
-
 // ---------------------------------------------------------------------------
 // sound-speed adaptation (viscosity.rs:89-100, eos.rs:74-81) and Tait EOS
 // ---------------------------------------------------------------------------
@@ -555,136 +317,6 @@ __global__ void k_eos(uint32_t N, const SceneDev* __restrict__ sc, State cur, Sc
   }
   cur.B2[s] = make_float4(b.x, b.y, b.z, P);
   (void)n_scenes;
-}
-
-// ---------------------------------------------------------------------------
-// pressure + viscosity + Akinci boundary forces, gravity, kick and drift
-// (wcsph.rs:181-259, 331-342). Gather form: every fluid particle sums its own
-// pair terms; boundary particles gather their reaction force.
-// ---------------------------------------------------------------------------
-__global__ void k_force(uint32_t N, const Ctl* __restrict__ ctl, Ctl* ctlw, const SceneDev* __restrict__ sc,
-                        const SceneState* __restrict__ st, State cur, const uint32_t* __restrict__ LIST,
-                        const uint32_t* __restrict__ CNT, uint32_t cap, int integrate) {
-  uint32_t G = gridDim.x;
-  uint32_t blk = xcd_block(blockIdx.x, G);
-  uint32_t s = blk * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  uint32_t meta = cur.META[s];
-  cur.IDF[s] = cur.ID[s];
-  if (meta & kMetaBoundary) {
-    cur.ACC[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  const SceneDev& S = sc[scene_of(meta)];
-  const float4 ai = cur.A2[s];
-  const float4 bi = cur.B2[s];
-  // Separations are in position units; q = r * q_unit, grad_i W = r_ij(units) * gf with
-  // gf = -dW/dr / r(units), and mu = (h/unit) v.r / (r^2 + eps h^2/unit^2), which equals
-  // the metre form exactly up to rounding.
-  const float r2s = S.r2s_u, qu = S.q_unit, aoh = S.alpha * S.inv_h;
-  const uint32_t n = CNT[s];
-  const uint4* lq = list_quads(LIST, s, cap);
-  const float m = S.mass, h = S.h_u, vb = S.visc_beta, eh2 = S.eh2_u;
-  const float c = (float)st[scene_of(meta)].c;
-  const float vac = -S.visc_alpha * c;
-  const float Pi = bi.w, rhoi = ai.w;
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  const float4* __restrict__ A2p = cur.A2;
-  const float4* __restrict__ B2p = cur.B2;
-  walk_list<Nb2>(
-      lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], B2p[j]}; },
-      [&](uint32_t e, const Nb2& nb) {
-        const float4 aj = nb.a, bj = nb.b;
-        (void)e;
-        float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);  // x_j - x_i
-        float r2 = rx * rx + ry * ry + rz * rz;
-        float rinv = __builtin_amdgcn_rsqf(r2);
-        float r = r2 * rinv;
-        float gf = -cubic_dwdr(r * qu, aoh) * rinv;  // grad_i W = (x_j - x_i) gf
-        // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
-        // with one reciprocal of d1 * rho_bar; zero for separating pairs.
-        float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
-        float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
-        float d1 = r2 + eh2;
-        float d2 = 0.5f * (rhoi + aj.w);
-        float t = h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
-        float pi = t * (vac + vb * (t * d2));
-        pi = (vr < 0.0f) ? pi : 0.0f;
-        float sf_fluid = gf * (-m * ((Pi + bj.w) + pi));
-        float sf_bnd = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
-        float sfac = (aj.w > 0.0f) ? sf_fluid : sf_bnd;
-        sfac = (r2 < r2s && r2 > 0.0f) ? sfac : 0.0f;
-        ax += rx * sfac;
-        ay += ry * sfac;
-        az += rz * sfac;
-      });
-  ax += S.g[0];
-  ay += S.g[1];
-  az += S.g[2];
-  cur.ACC[s] = make_float4(ax, ay, az, 0.0f);
-  if (!integrate) return;
-  const double dt = ctl->dt;
-  const float dtf = (float)dt;
-  float vx = bi.x + ax * dtf, vy = bi.y + ay * dtf, vz = bi.z + az * dtf;
-  if (S.dim == 2) vz = 0.0f;
-  double x = cur.X[s] + (double)vx * dt;
-  double y = cur.Y[s] + (double)vy * dt;
-  double z = (S.dim == 2) ? 0.0 : cur.Z[s] + (double)vz * dt;
-  cur.X[s] = x;
-  cur.Y[s] = y;
-  cur.Z[s] = z;
-  cur.A[s] = make_float4(__int_as_float(to_units(x, S.org[0], S.inv_unit)),
-                         __int_as_float(to_units(y, S.org[1], S.inv_unit)),
-                         __int_as_float(to_units(z, S.org[2], S.inv_unit)), m);
-  cur.B[s] = make_float4(vx, vy, vz, 0.0f);
-  float d = cur.DISP[s] + sqrtf(vx * vx + vy * vy + vz * vz) * dtf;
-  cur.DISP[s] = d;
-  unsigned long long over = __ballot(!(d <= S.skin_half));  // also catches NaN; active lanes only
-  if (over != 0ull && (int)(threadIdx.x & (kWave - 1)) == __ffsll((long long)over) - 1) ctlw->need_rebuild = 1u;
-}
-
-// Akinci reaction on boundary samples (wcsph.rs:253-254): each boundary
-// particle gathers -m f over the fluid particles that see it.
-__global__ void k_boundary_force(uint32_t N, const SceneDev* __restrict__ sc, State cur,
-                                 const uint32_t* __restrict__ LIST, const uint32_t* __restrict__ CNT, uint32_t cap,
-                                 int with_boundary) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  uint32_t meta = cur.META[s];
-  if (!(meta & kMetaBoundary)) {
-    cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  if (with_boundary) {
-    const SceneDev& S = sc[scene_of(meta)];
-    const float4 ai = cur.A2[s];
-    const float r2s = S.r2s_u, qu = S.q_unit, aoh = S.alpha * S.inv_h;
-    const float nm = -S.mass;
-    const uint32_t n = CNT[s];
-    const uint4* lq = list_quads(LIST, s, cap);
-    const float4* __restrict__ A2p = cur.A2;
-    const float4* __restrict__ B2p = cur.B2;
-    walk_list<Nb2>(
-        lq, n, N, [&](uint32_t j) { return Nb2{A2p[j], B2p[j]}; },
-        [&](uint32_t e, const Nb2& nb) {
-          const float4 aj = nb.a;
-          const float Pj = nb.b.w;
-          (void)e;
-          // from the fluid particle's view: r = x_b - x_f
-          float rx = units_diff(ai.x, aj.x), ry = units_diff(ai.y, aj.y), rz = units_diff(ai.z, aj.z);
-          float r2 = rx * rx + ry * ry + rz * rz;
-          float rinv = __builtin_amdgcn_rsqf(r2);
-          float r = r2 * rinv;
-          float gf = -cubic_dwdr(r * qu, aoh) * rinv;
-          float sfac = gf * (ai.w * Pj) * nm;  // gw * (-(rho0 psi_b) P_f) * (-m)
-          sfac = (r2 < r2s && r2 > 0.0f) ? sfac : 0.0f;
-          fx += rx * sfac;
-          fy += ry * sfac;
-          fz += rz * sfac;
-        });
-  }
-  cur.BF[s] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -955,26 +587,6 @@ __global__ void k_probe_candidates(const SceneDev* __restrict__ sc, uint32_t sce
   *out_len = n;
 }
 
-// The device list of the particle in `slot`, filtered to r < 2h, as original
-// indices (unsorted; the host orders them for presentation).
-__global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, State cur,
-                               const uint32_t* __restrict__ LIST, const uint32_t* __restrict__ CNT, uint32_t cap,
-                               uint64_t* __restrict__ out, uint32_t* __restrict__ out_len) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const SceneDev& S = sc[scene_of(cur.META[slot])];
-  float4 ai = cur.A[slot];
-  uint32_t n = CNT[slot], m = 0;
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t e = LIST[list_index(slot, k, cap)];
-    uint32_t j = e & kSlotMask;
-    float4 aj = cur.A[j];
-    float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
-    float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 < S.r2s_u) out[m++] = cur.ID[j];
-  }
-  *out_len = m;
-}
-
 // slot of each original index
 __global__ void k_slot_of_id(uint32_t N, State cur, uint32_t* __restrict__ slot_of_id) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1045,21 +657,12 @@ void launch_finalize(hipStream_t st, uint32_t N, const Ctl* ctl, const SceneDev*
   PEF_LAUNCH(k_copyback, blocks_for(nmax, bs), bs, st, N, ctl, alt, cur, counts, C);
 }
 
-void launch_nlist(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc,
-                  const uint32_t* cell_start, const State& cur, uint32_t* LIST, uint32_t* CNT, uint32_t cap,
-                  int with_boundary, int always, uint32_t bs) {
-  PEF_LAUNCH(k_nlist, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, cell_start, cur, LIST, CNT, cap, with_boundary,
-             always);
-}
-
-void launch_density(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                    const uint32_t* CNT, uint32_t cap, uint32_t bs) {
-  PEF_LAUNCH(k_density, blocks_for(N, bs), bs, st, N, sc, cur, LIST, CNT, cap);
-}
-
-void launch_xsph(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                 const uint32_t* CNT, uint32_t cap, uint32_t bs) {
-  PEF_LAUNCH(k_xsph, blocks_for(N, bs), bs, st, N, sc, cur, LIST, CNT, cap);
+void launch_scan(hipStream_t st, const Ctl* ctl, int always, const uint32_t* in, uint32_t* out,
+                 uint32_t* block_sums, uint32_t n) {
+  const uint32_t nb = blocks_for(n, kScanTile);
+  PEF_LAUNCH(k_scan_local, nb, kScanThreads, st, ctl, always, in, out, block_sums, n);
+  PEF_LAUNCH(k_scan_sums, 1, kScanThreads, st, ctl, always, block_sums, nb);
+  PEF_LAUNCH(k_scan_add, blocks_for(n + 1, 256), 256, st, ctl, always, out, block_sums, n, nb);
 }
 
 void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, SceneState* ss,
@@ -1069,13 +672,6 @@ void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State
   PEF_LAUNCH(k_vmax_zero, blocks_for(n_scenes, 64), 64, st, ss, n_scenes);
   PEF_LAUNCH(k_vmax, blocks_for(N, bs), bs, st, N, cur.B2, cur.META, ss);
   PEF_LAUNCH(k_eos, blocks_for(N, bs), bs, st, N, sc, cur, ss, n_scenes);
-}
-
-void launch_force(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc, const SceneState* ss,
-                  const State& cur, const uint32_t* LIST, const uint32_t* CNT, uint32_t cap, int integrate,
-                  int with_boundary, uint32_t bs) {
-  PEF_LAUNCH(k_force, blocks_for(N, bs), bs, st, N, ctl, ctlw, sc, ss, cur, LIST, CNT, cap, integrate);
-  PEF_LAUNCH(k_boundary_force, blocks_for(N, 256), 256, st, N, sc, cur, LIST, CNT, cap, with_boundary);
 }
 
 void launch_diag(hipStream_t st, uint32_t first, uint32_t count, const SceneDev* sc, uint32_t scene, const State& cur,
@@ -1122,11 +718,6 @@ void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene,
                              const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
                              uint64_t* out_len) {
   PEF_LAUNCH(k_probe_candidates, 1, 64, st, sc, scene, key, cell_start, order, out, cap, out_len);
-}
-
-void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                         const uint32_t* CNT, uint32_t cap, uint64_t* out, uint32_t* out_len) {
-  PEF_LAUNCH(k_extract_list, 1, 64, st, slot, sc, cur, LIST, CNT, cap, out, out_len);
 }
 
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id) {

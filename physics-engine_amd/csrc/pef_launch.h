@@ -1,4 +1,5 @@
-// Host-side launchers of the WCSPH kernels (defined in pef_kernels.hip).
+// Host-side launchers of the WCSPH kernels (defined in pef_kernels.hip; the
+// list and pair kernels are declared in pef_pair.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,18 +34,12 @@ void launch_grid_build(hipStream_t st, const GridArgs& g, int always);
 void launch_finalize(hipStream_t st, uint32_t N, const Ctl* ctl, const SceneDev* sc, const uint32_t* cell_of,
                      const uint32_t* cell_start, const uint32_t* perm, const State& cur, const State& alt,
                      uint32_t* counts, uint32_t C);
-void launch_nlist(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc,
-                  const uint32_t* cell_start, const State& cur, uint32_t* LIST, uint32_t* CNT, uint32_t cap,
-                  int with_boundary, int always, uint32_t bs);
-void launch_density(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                    const uint32_t* CNT, uint32_t cap, uint32_t bs);
-void launch_xsph(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                 const uint32_t* CNT, uint32_t cap, uint32_t bs);
+// exclusive scan of n counters into out[0..n] (out[n] = total), gated on a rebuild
+// unless `always`
+void launch_scan(hipStream_t st, const Ctl* ctl, int always, const uint32_t* in, uint32_t* out,
+                 uint32_t* block_sums, uint32_t n);
 void launch_pressure(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, SceneState* ss,
                      uint32_t n_scenes, uint32_t* unused);
-void launch_force(hipStream_t st, uint32_t N, const Ctl* ctl, Ctl* ctlw, const SceneDev* sc, const SceneState* ss,
-                  const State& cur, const uint32_t* LIST, const uint32_t* CNT, uint32_t cap, int integrate,
-                  int with_boundary, uint32_t bs);
 void launch_diag(hipStream_t st, uint32_t first, uint32_t count, const SceneDev* sc, uint32_t scene, const State& cur,
                  DiagAcc* part, uint32_t nb, DiagAcc* out);
 void launch_unsort(hipStream_t st, uint32_t N, const SceneDev* sc, const SceneState* ss, const State& cur, int what,
@@ -62,8 +57,6 @@ void launch_compact_cells(hipStream_t st, uint32_t C, const SceneDev* sc, uint32
 void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene, const int32_t* key,
                              const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
                              uint64_t* out_len);
-void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint32_t* LIST,
-                         const uint32_t* CNT, uint32_t cap, uint64_t* out, uint32_t* out_len);
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id);
 void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl);
 void launch_calib_copy(hipStream_t st, const float4* src, float4* dst, uint64_t n);

@@ -1,0 +1,61 @@
+// Host interface of the brick-tiled list and pair kernels (pef_pair.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pef_launch.h"
+
+namespace pef {
+
+enum { kTileNlist = 0, kTileDensity = 1, kTileXsph = 2, kTileForce = 3, kTileKinds = 4 };
+
+// Bytes of one brick's halo table (slot runs and prefix of its 48 halo cells).
+constexpr uint32_t kHaloTabBytes = 448;
+
+// Kernel arguments shared by the tiled kernels.
+struct TileArgs {
+  const uint32_t* cell_start;  // list grid, size C + 1
+  const uint32_t* bricks;      // global ids of the non-empty bricks
+  const uint32_t* n_bricks;    // their number (device)
+  const void* tables;          // halo table of each non-empty brick (kHaloTabBytes each)
+  uint32_t* queue;             // 16 words per kernel kind: 8 range counters, finished-workgroup counter
+  uint16_t* LIST;
+  uint32_t* CNT;
+  uint32_t cap;      // list entries per particle (multiple of 4)
+  uint32_t lds_cap;  // halo particles the kernel stages into LDS (set per kernel by the launcher)
+  uint32_t N;        // sentinel slot
+};
+
+// Launch configuration, filled by tiles_configure from the device limits.
+struct TileConfig {
+  uint32_t bs = 1024;       // threads per workgroup, pair kernels
+  uint32_t nlist_bs = 512;  // threads per workgroup, list build
+  uint32_t lpp = 8;         // lanes per particle, pair kernels (2, 4, 8, 16)
+  uint32_t lds_cap_limit = 0;  // optional upper bound on every kernel's LDS halo capacity
+  uint32_t n_cu = 0, max_lds = 0;
+  uint32_t lds_cap[kTileKinds] = {};
+  uint32_t per_cu[kTileKinds] = {};
+  uint32_t grid[kTileKinds] = {};
+};
+
+// 0 on success; 1 device query failed, 2 bad configuration, 3 launch attribute failed.
+int tiles_configure(TileConfig& c);
+
+void launch_brick_list(hipStream_t st, uint32_t NB, const Ctl* ctl, int always, const uint32_t* cell_start,
+                       uint32_t* flag, uint32_t* pos, uint32_t* block_sums, uint32_t* bricks, const SceneDev* sc,
+                       const uint32_t* META, void* tables);
+void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
+                        const SceneDev* sc, const State& cur, int with_boundary, int always);
+void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
+                          const State& cur);
+void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
+                       const State& cur);
+void launch_force_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
+                        const SceneDev* sc, const SceneState* ss, const State& cur, int integrate,
+                        int with_boundary);
+void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint16_t* LIST,
+                         const uint32_t* CNT, uint32_t cap, const uint32_t* cell_start, uint64_t* out,
+                         uint32_t* out_len);
+
+}  // namespace pef

@@ -1,0 +1,923 @@
+// Brick-tiled neighbour lists and pair kernels of the WCSPH step (gfx950).
+//
+// Reference function -> kernel:
+//   UniformGrid::neighbours (grid.rs:122-140), filtered  -> k_nlist_tiled
+//   WcsphWorld::compute_density (wcsph.rs:117-154)      -> k_density_tiled
+//   apply_xsph (wcsph.rs:283-323)                        -> k_xsph_tiled
+//   compute_forces + kick/drift (wcsph.rs:181-342)       -> k_force_tiled
+//
+// Tiling. The list grid's cells have edge 2h (1 + skin), so every list partner
+// of a particle lies in its own cell or one of the 26 adjacent cells. Cells are
+// numbered brick-major, 2 x 2 x 1 cells per brick, so the particles of a brick
+// are one contiguous slot range. Every kernel here is persistent: a workgroup
+// takes bricks from a work queue and, for each brick,
+//   1. wave 0 reads the slot ranges of the brick's halo (the 4 x 4 x 3 cells a
+//      brick particle can reach) and prefix-sums their counts into halo-local
+//      indices;
+//   2. the workgroup copies the halo particles (one contiguous slot run per
+//      cell) from HBM into LDS with coalesced loads;
+//   3. each brick particle walks its list, reading every partner from LDS.
+// The list build uses the same halo, so a list entry is a 16-bit halo-local
+// index and no pair kernel issues a scattered global load for a partner.
+//
+// A halo larger than a kernel's LDS capacity runs the same code with partner
+// loads from global memory (the slot of a halo-local index comes from the
+// halo table); both paths evaluate the same arithmetic in the same order, so
+// results do not depend on which path ran.
+//
+// Work queues: the compacted list of non-empty bricks is cut into eight
+// contiguous ranges, one per XCD (workgroups are dealt to the XCDs round robin,
+// so workgroup b runs on XCD b % 8). A workgroup drains its own XCD's range
+// first, so neighbouring bricks, which share halo cells, are staged through the
+// same L2, then helps with the other ranges. The last workgroup to finish
+// resets the counters for the next launch.
+//
+// Several lanes per particle: a workgroup's threads are grouped LPP to a brick
+// particle; lane `sub` of a group walks list words sub, sub + LPP, ... and the
+// partial sums are combined by a fixed butterfly, so the result is
+// deterministic. This keeps the waves busy when a brick's particle count is not
+// a multiple of the workgroup size.
+#include "pef_common.h"
+#include "pef_launch.h"
+#include "pef_pair.h"
+
+namespace pef {
+
+constexpr uint32_t kHaloCells = 48;  // 4 x 4 x 3
+constexpr uint32_t kNoBrick = 0xFFFFFFFFu;
+
+// Halo table of a brick. Built once per list build for every non-empty brick
+// (k_brick_tables) and copied into LDS by the kernels that process the brick.
+struct __align__(16) HaloTab {
+  uint32_t start[kHaloCells];    // first slot of each halo cell
+  uint32_t pre[kHaloCells + 1];  // halo-local index of each cell's first particle; pre[48] = halo size
+  uint32_t own_end[4];           // slot end of each of the brick's own cells
+  uint32_t gb;                   // global brick index, kNoBrick when the queues are empty
+  uint32_t scene;
+  uint32_t s0, s1;               // the brick's slot range
+  uint32_t pad[7];
+};
+static_assert(sizeof(HaloTab) == kHaloTabBytes, "halo table size");
+constexpr uint32_t kTabQuads = sizeof(HaloTab) / 16;
+
+// Halo cell index of halo coordinates (hx, hy, hz) in [0,4) x [0,4) x [0,3).
+__device__ inline uint32_t halo_k(uint32_t hx, uint32_t hy, uint32_t hz) { return hx * 12u + hy * 3u + hz; }
+// Halo cell of own cell q = 2 ix + iy of the brick: (ix + 1, iy + 1, 1).
+__device__ inline uint32_t own_k(uint32_t q) { return halo_k((q >> 1) + 1u, (q & 1u) + 1u, 1u); }
+
+// Halo cell holding halo-local index t (t < halo size): the last cell whose
+// first index is <= t (empty cells are skipped because their successor has
+// the same first index).
+__device__ inline uint32_t halo_cell_of(const HaloTab& T, uint32_t t) {
+  uint32_t lo = 0, hi = kHaloCells;  // T.pre[lo] <= t < T.pre[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (T.pre[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__device__ inline uint32_t halo_slot(const HaloTab& T, uint32_t t) {
+  const uint32_t k = halo_cell_of(T, t);
+  return T.start[k] + (t - T.pre[k]);
+}
+
+// One wave (all 64 lanes, lane = threadIdx.x) fills the halo table of global
+// brick gb, which holds at least one particle.
+__device__ inline void halo_setup(HaloTab& T, uint32_t gb, const SceneDev* __restrict__ sc,
+                                  const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ META) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t s0 = cell_start[gb * kBrickCells], s1 = cell_start[gb * kBrickCells + kBrickCells];
+  const uint32_t scene = scene_of(META[s0]);
+  const SceneDev& S = sc[scene];
+  const uint32_t lb = gb - S.cell_base / kBrickCells;
+  const uint32_t bz = lb % (uint32_t)S.nz, t = lb / (uint32_t)S.nz;
+  const uint32_t by = t % (uint32_t)S.nby, bx = t / (uint32_t)S.nby;
+  uint32_t cnt = 0, st = 0;
+  if (lane < kHaloCells) {
+    const int32_t cx = (int32_t)(2u * bx) - 1 + (int32_t)(lane / 12u);
+    const int32_t cy = (int32_t)(2u * by) - 1 + (int32_t)((lane / 3u) & 3u);
+    const int32_t cz = (int32_t)bz - 1 + (int32_t)(lane % 3u);
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < S.nx && cy < S.ny && cz < S.nz) {
+      const uint32_t c = cell_id(S, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+      st = cell_start[c];
+      cnt = cell_start[c + 1] - st;
+    }
+  }
+  uint32_t x = cnt;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, kWave);
+    if ((int)lane >= o) x += y;
+  }
+  if (lane < kHaloCells) {
+    T.start[lane] = st;
+    T.pre[lane] = x - cnt;
+  }
+  if (lane == kHaloCells - 1) T.pre[kHaloCells] = x;
+  if (lane < 4) T.own_end[lane] = cell_start[gb * kBrickCells + lane + 1];
+  if (lane == 0) {
+    T.gb = gb;
+    T.scene = scene;
+    T.s0 = s0;
+    T.s1 = s1;
+  }
+}
+
+// Next brick for a workgroup on XCD x0; `tries` counts the exhausted ranges.
+__device__ inline uint32_t queue_take(uint32_t* __restrict__ q, uint32_t nb, uint32_t x0, uint32_t& tries) {
+  while (tries < 8u) {
+    const uint32_t qi = (x0 + tries) & 7u;
+    const uint32_t lo = (uint32_t)(((uint64_t)nb * qi) >> 3);
+    const uint32_t hi = (uint32_t)(((uint64_t)nb * (qi + 1u)) >> 3);
+    if (hi > lo) {
+      const uint32_t t = atomicAdd(&q[qi], 1u);
+      if (t < hi - lo) return lo + t;
+    }
+    ++tries;
+  }
+  return kNoBrick;
+}
+
+// Persistent brick loop; work(T) runs once per brick with its table in LDS
+// and must be called by every thread (it may contain barriers). The queue
+// index of the next brick is taken while the current one is processed, so
+// the only wait at a brick change is one coalesced 448-byte table load.
+template <class Work>
+__device__ inline void brick_loop(const TileArgs& ta, HaloTab& T, Work work) {
+  uint32_t tries = 0, next = kNoBrick;
+  const uint32_t nb = *ta.n_bricks;
+  const uint32_t x0 = blockIdx.x & 7u;
+  const uint4* __restrict__ tabs = reinterpret_cast<const uint4*>(ta.tables);
+  if (threadIdx.x == 0) next = queue_take(ta.queue, nb, x0, tries);
+  while (true) {
+    if (threadIdx.x < kWave) {
+      const uint32_t cur = __shfl(next, 0, kWave);
+      if (cur != kNoBrick) {
+        if (threadIdx.x == 0) next = queue_take(ta.queue, nb, x0, tries);
+        if (threadIdx.x < kTabQuads) reinterpret_cast<uint4*>(&T)[threadIdx.x] = tabs[(uint64_t)cur * kTabQuads + threadIdx.x];
+      } else if (threadIdx.x == 0) {
+        T.gb = kNoBrick;
+      }
+    }
+    __syncthreads();
+    if (T.gb == kNoBrick) break;
+    work(T);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ta.queue[8], 1u) == gridDim.x - 1u) {
+      for (int i = 0; i < 8; ++i) atomicExch(&ta.queue[i], 0u);
+      atomicExch(&ta.queue[8], 0u);
+    }
+  }
+}
+
+// Copies the halo's particles into LDS (one or two arrays). Each thread issues
+// the loads of up to kStageUnroll particles before it writes any of them, so
+// a brick change costs about one memory latency rather than one per particle.
+constexpr uint32_t kStageUnroll = 8;
+template <int NARR>
+__device__ inline void stage_halo(const HaloTab& T, uint32_t H, float4* __restrict__ LA, float4* __restrict__ LB,
+                                  const float4* __restrict__ GA, const float4* __restrict__ GB) {
+  for (uint32_t t0 = threadIdx.x; t0 < H; t0 += kStageUnroll * blockDim.x) {
+    float4 a[kStageUnroll], b[kStageUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kStageUnroll; ++u) {
+      const uint32_t t = t0 + u * blockDim.x;
+      if (t < H) {
+        const uint32_t s = halo_slot(T, t);
+        a[u] = GA[s];
+        if (NARR == 2) b[u] = GB[s];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kStageUnroll; ++u) {
+      const uint32_t t = t0 + u * blockDim.x;
+      if (t < H) {
+        LA[t] = a[u];
+        if (NARR == 2) LB[t] = b[u];
+      }
+    }
+  }
+}
+
+// Slot, halo-local index and own-cell number of the brick's p-th particle.
+struct Own {
+  uint32_t s, local, q;
+};
+__device__ inline Own own_of(const HaloTab& T, uint32_t p) {
+  Own o;
+  o.s = T.s0 + p;
+  o.q = (o.s >= T.own_end[0] ? 1u : 0u) + (o.s >= T.own_end[1] ? 1u : 0u) + (o.s >= T.own_end[2] ? 1u : 0u);
+  const uint32_t k = own_k(o.q);
+  o.local = T.pre[k] + (o.s - T.start[k]);
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// list storage: u16 entries, contiguous per slot (LIST[s * cap + k]), read as
+// 8-byte words of four entries
+// ---------------------------------------------------------------------------
+__device__ inline const uint2* list_words(const uint16_t* LIST, uint32_t s, uint32_t cap) {
+  return reinterpret_cast<const uint2*>(LIST + (uint64_t)s * cap);
+}
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+// Lists are streamed once per pass: non-temporal loads keep them from evicting
+// the halo data that neighbouring bricks stage through L2.
+__device__ inline uint2 load_word(const uint2* lw, uint32_t g) {
+  const u32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(lw + g));
+  return make_uint2(v.x, v.y);
+}
+
+// Walk list words sub, sub + LPP, ... < ng; fn(word) evaluates four entries.
+// The load of the next word is issued before the current one is evaluated.
+template <int LPP, class Fn>
+__device__ inline void walk_words(const uint2* lw, uint32_t ng, uint32_t sub, Fn fn) {
+  if (sub >= ng) return;
+  uint2 cur = load_word(lw, sub);
+  for (uint32_t g = sub; g < ng; g += LPP) {
+    const uint32_t gn = g + LPP;
+    const uint2 nxt = (gn < ng) ? load_word(lw, gn) : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    fn(cur);
+    cur = nxt;
+  }
+}
+
+template <int LPP>
+__device__ inline float group_sum(float v) {
+#pragma unroll
+  for (int o = 1; o < LPP; o <<= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Partner state source: LDS (FAST) or global memory through the halo table.
+template <bool FAST, int NARR>
+struct Src {
+  const float4* LA;
+  const float4* LB;
+  uint32_t cap;  // LDS sentinel index
+  const float4* GA;
+  const float4* GB;
+  const HaloTab* T;
+  uint32_t N;  // global sentinel slot
+  __device__ inline void get(uint32_t e, float4& a, float4& b) const {
+    if (FAST) {
+      const uint32_t i = e < cap ? e : cap;  // list padding -> sentinel
+      a = LA[i];
+      if (NARR == 2) b = LB[i];
+    } else {
+      const uint32_t s = (e < T->pre[kHaloCells]) ? halo_slot(*T, e) : N;  // padding -> sentinel slot
+      a = GA[s];
+      if (NARR == 2) b = GB[s];
+    }
+  }
+};
+
+__device__ inline void sentinel_lds(float4* LA, float4* LB, uint32_t cap, int narr) {
+  if (threadIdx.x == 0) {
+    const float far = __int_as_float(kSentinelUnits);
+    LA[cap] = make_float4(far, far, far, 0.0f);
+    if (narr == 2) LB[cap] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour-list build (grid.rs:122-140 candidates, filtered to the list radius)
+// ---------------------------------------------------------------------------
+// One-sided Akinci filter (particles.rs:126-134) in FP64, evaluated like the
+// reference with no contraction.
+__device__ inline bool sees_f64(double bx, double by, double bz, double nx, double ny, double nz, double fx,
+                                double fy, double fz, double h) {
+#pragma clang fp contract(off)
+  double n2 = nx * nx + ny * ny + nz * nz;
+  if (n2 <= 0.0) return true;
+  double eps = 0.15 * h * sqrt(n2 > 0.0 ? n2 : 0.0);
+  double d = (fx - bx) * nx + (fy - by) * ny + (fz - bz) * nz;
+  return d > -eps;
+}
+
+// Slot of halo-local index t inside the three-cell z-row starting at cell k0.
+__device__ inline uint32_t row_slot(const HaloTab& T, uint32_t k0, uint32_t t) {
+  const uint32_t k = k0 + (t >= T.pre[k0 + 1] ? 1u : 0u) + (t >= T.pre[k0 + 2] ? 1u : 0u);
+  return T.start[k] + (t - T.pre[k]);
+}
+
+// List build with kNlistLanes lanes per particle: the lanes of a group test
+// consecutive candidates of a z-row together, and a ballot ranks the accepted
+// ones, so every list is written in ascending halo-local order with no
+// divergent per-lane append loop.
+constexpr uint32_t kNlistLanes = 16;
+constexpr uint64_t kNlistGroupMask = (1ull << kNlistLanes) - 1ull;
+
+template <bool FAST>
+__device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, const Own& o, uint32_t sub,
+                                       const float4* LA, State& cur, uint16_t* LIST, uint32_t cap) {
+  const uint32_t s = o.s;
+  const uint32_t meta = cur.META[s];
+  const bool i_bnd = (meta & kMetaBoundary) != 0u;
+  const bool i_or = (meta & kMetaOriented) != 0u;
+  const float4 ai = FAST ? LA[o.local] : cur.A[s];
+  const float r2l = S.r2l_u;
+  const uint32_t hx = (o.q >> 1) + 1u, hy = (o.q & 1u) + 1u;
+  uint16_t* out = LIST + (uint64_t)s * cap;
+  const uint32_t gshift = threadIdx.x & (uint32_t)(kWave - 1) & ~(kNlistLanes - 1u);  // group's first lane
+  const uint64_t below = (1ull << sub) - 1ull;
+  uint32_t cnt = 0;
+  // z-rows of three cells in ascending halo order, so the list is ascending
+  for (uint32_t dx = 0; dx < 3; ++dx) {
+    for (uint32_t dy = 0; dy < 3; ++dy) {
+      const uint32_t k0 = halo_k(hx + dx - 1u, hy + dy - 1u, 0u);
+      const uint32_t t0 = T.pre[k0], t1 = T.pre[k0 + 3];
+      for (uint32_t tb = t0; tb < t1; tb += kNlistLanes) {
+        const uint32_t t = tb + sub;
+        bool ok = false;
+        if (t < t1) {
+          float4 aj;
+          if (FAST) aj = LA[t]; else aj = cur.A[row_slot(T, k0, t)];
+          const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          ok = (r2 < r2l) && t != o.local;
+          if (ok) {
+            const bool j_bnd = is_boundary_w(aj.w);
+            if (!i_bnd) {
+              if (j_bnd && S.oriented) {
+                const uint32_t j = row_slot(T, k0, t);
+                if (cur.META[j] & kMetaOriented)
+                  ok = sees_f64(cur.X[j], cur.Y[j], cur.Z[j], cur.NX[j], cur.NY[j], cur.NZ[j], cur.X[s], cur.Y[s],
+                                cur.Z[s], S.h_d);
+              }
+            } else {
+              ok = !j_bnd;
+              if (ok && i_or) {
+                const uint32_t j = row_slot(T, k0, t);
+                ok = sees_f64(cur.X[s], cur.Y[s], cur.Z[s], cur.NX[s], cur.NY[s], cur.NZ[s], cur.X[j], cur.Y[j],
+                              cur.Z[j], S.h_d);
+              }
+            }
+          }
+        }
+        const uint64_t m = (__ballot(ok) >> gshift) & kNlistGroupMask;
+        if (ok) {
+          const uint32_t k = cnt + (uint32_t)__popcll(m & below);
+          if (k < cap) out[k] = (uint16_t)t;
+        }
+        cnt += (uint32_t)__popcll(m);
+      }
+    }
+  }
+  const uint32_t stored = cnt < cap ? cnt : cap;
+  const uint32_t padn = (4u - (stored & 3u)) & 3u;  // pad the last word
+  if (sub < padn) out[stored + sub] = (uint16_t)kListPad;
+  return cnt;
+}
+
+__global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __restrict__ ctl, Ctl* ctlw,
+                                                      const SceneDev* __restrict__ sc, State cur, int with_boundary,
+                                                      int always) {
+  if (!always && !ctl->rebuilding) return;  // uniform across the grid
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  float4* LA = lds;
+  const uint32_t CAP = ta.lds_cap;
+  sentinel_lds(LA, nullptr, CAP, 1);
+  uint32_t mymax = 0;
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    const bool encodable = H <= kListPad;  // indices < H must stay below the pad value
+    if (fast) stage_halo<1>(Tb, H, LA, nullptr, cur.A, nullptr);
+    if (threadIdx.x == 0) {
+      if (!fast) atomicAdd(&ctlw->slow_bricks, 1u);
+      if (!encodable) atomicOr(&ctlw->flags, kFlagNeighbours);
+    }
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / kNlistLanes, sub = threadIdx.x % kNlistLanes;
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      const uint32_t p = base + threadIdx.x / kNlistLanes;
+      if (p < n_own) {  // uniform within a lane group
+        const Own o = own_of(Tb, p);
+        uint32_t cnt = 0;
+        const bool i_bnd = (cur.META[o.s] & kMetaBoundary) != 0u;
+        if (encodable && (!i_bnd || with_boundary))
+          cnt = fast ? nlist_group<true>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap)
+                     : nlist_group<false>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap);
+        if (sub == 0u) ta.CNT[o.s] = cnt < ta.cap ? cnt : ta.cap;
+        mymax = mymax > cnt ? mymax : cnt;
+      }
+    }
+  });
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint32_t t2 = __shfl_xor(mymax, o, kWave);
+    mymax = mymax > t2 ? mymax : t2;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && mymax) {
+    atomicMax(&ctlw->max_count, mymax);
+    if (mymax > ta.cap) atomicOr(&ctlw->flags, kFlagNeighbours);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// density summation with compensated FP32 sums (wcsph.rs:117-154)
+// ---------------------------------------------------------------------------
+template <int LPP, bool FAST>
+__device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, float r2s,
+                                     float qu, const Src<FAST, 1>& src) {
+  float sum = 0.0f, comp = 0.0f;
+  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
+    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
+    float4 a[4], unused;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], unused);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float4 aj = a[k];
+      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float r = __builtin_amdgcn_sqrtf(r2);
+      const float w = fabsf(aj.w) * cubic_w(r * qu, 1.0f);  // weight m (fluid) or rho0 psi (boundary)
+      const float term = (r2 < r2s) ? w : 0.0f;
+      const float y = term - comp;  // compensated (Kahan) sum
+      const float tt = sum + y;
+      comp = (tt - sum) - y;
+      sum = tt;
+    }
+  });
+  return sum - comp;
+}
+
+template <int LPP>
+__global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const SceneDev* __restrict__ sc, State cur) {
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  float4* LA = lds;
+  const uint32_t CAP = ta.lds_cap;
+  sentinel_lds(LA, nullptr, CAP, 1);
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    if (fast) stage_halo<1>(Tb, H, LA, nullptr, cur.A, nullptr);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const float r2s = S.r2s_u, qu = S.q_unit;
+    const Src<true, 1> fsrc{LA, nullptr, CAP, cur.A, nullptr, &Tb, ta.N};
+    const Src<false, 1> gsrc{LA, nullptr, CAP, cur.A, nullptr, &Tb, ta.N};
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      const uint32_t p = base + threadIdx.x / LPP;
+      const bool act = p < n_own;
+      Own o{0, 0, 0};
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float part = 0.0f;
+      if (act) {
+        o = own_of(Tb, p);
+        ai = fast ? LA[o.local] : cur.A[o.s];
+        if (!is_boundary_w(ai.w)) {
+          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
+          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
+          part = fast ? density_walk<LPP, true>(lw, ng, sub, ai, r2s, qu, fsrc)
+                      : density_walk<LPP, false>(lw, ng, sub, ai, r2s, qu, gsrc);
+        }
+      }
+      const float tot = group_sum<LPP>(part);
+      if (act && sub == 0u) {
+        if (is_boundary_w(ai.w)) {
+          cur.A2[o.s] = ai;  // boundary keeps w = -rho0 psi; its density is rho0 (wcsph.rs:126-129)
+        } else {
+          const float rho = fmaxf(S.alpha * (S.mass + tot), 1e-12f);  // self term m W(0) = m alpha
+          cur.A2[o.s] = make_float4(ai.x, ai.y, ai.z, rho);
+        }
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// XSPH velocity smoothing (wcsph.rs:283-323), Jacobi: reads B, writes B2
+// ---------------------------------------------------------------------------
+template <int LPP, bool FAST>
+__device__ inline float3 xsph_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, const float4& bi,
+                                   float r2s, float qu, float alpha, float two_m, const Src<FAST, 2>& src) {
+  float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
+    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
+    float4 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], b[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float4 aj = a[k], bj = b[k];
+      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float r = __builtin_amdgcn_sqrtf(r2);
+      const float wij = cubic_w(r * qu, alpha);
+      // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2
+      float sfac = two_m * wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
+      sfac = (r2 < r2s && aj.w > 0.0f) ? sfac : 0.0f;  // fluid partners only
+      dvx += (bj.x - bi.x) * sfac;
+      dvy += (bj.y - bi.y) * sfac;
+      dvz += (bj.z - bi.z) * sfac;
+    }
+  });
+  return make_float3(dvx, dvy, dvz);
+}
+
+template <int LPP>
+__global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev* __restrict__ sc, State cur) {
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  const uint32_t CAP = ta.lds_cap;
+  float4* LA = lds;
+  float4* LB = lds + CAP + 1;
+  sentinel_lds(LA, LB, CAP, 2);
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    if (fast) stage_halo<2>(Tb, H, LA, LB, cur.A2, cur.B);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha, two_m = 2.0f * S.mass, eps = S.xsph_eps;
+    const Src<true, 2> fsrc{LA, LB, CAP, cur.A2, cur.B, &Tb, ta.N};
+    const Src<false, 2> gsrc{LA, LB, CAP, cur.A2, cur.B, &Tb, ta.N};
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      const uint32_t p = base + threadIdx.x / LPP;
+      const bool act = p < n_own;
+      Own o{0, 0, 0};
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
+      float3 dv = make_float3(0.0f, 0.0f, 0.0f);
+      bool smooth = false;
+      if (act) {
+        o = own_of(Tb, p);
+        ai = fast ? LA[o.local] : cur.A2[o.s];
+        bi = fast ? LB[o.local] : cur.B[o.s];
+        smooth = !is_boundary_w(ai.w) && eps != 0.0f;
+        if (smooth) {
+          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
+          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
+          dv = fast ? xsph_walk<LPP, true>(lw, ng, sub, ai, bi, r2s, qu, alpha, two_m, fsrc)
+                    : xsph_walk<LPP, false>(lw, ng, sub, ai, bi, r2s, qu, alpha, two_m, gsrc);
+        }
+      }
+      dv.x = group_sum<LPP>(dv.x);
+      dv.y = group_sum<LPP>(dv.y);
+      dv.z = group_sum<LPP>(dv.z);
+      if (act && sub == 0u) {
+        const float Pold = cur.B2[o.s].w;
+        float vx = bi.x, vy = bi.y, vz = bi.z;
+        if (smooth) {
+          vx += dv.x * eps;
+          vy += dv.y * eps;
+          vz += dv.z * eps;
+          if (S.dim == 2) vz = 0.0f;
+        }
+        cur.B2[o.s] = make_float4(vx, vy, vz, Pold);
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// pressure + viscosity + Akinci boundary forces, gravity, kick and drift
+// (wcsph.rs:181-259, 331-342). Gather form: a fluid particle sums its own pair
+// terms; a boundary particle sums the reaction -m f of the fluid particles
+// that see it (wcsph.rs:253-254). Both run the same pair body, so a wave that
+// holds both kinds does not diverge.
+// ---------------------------------------------------------------------------
+struct ForceConsts {
+  float r2s, qu, aoh, m, h, vb, eh2, vac;
+};
+
+template <int LPP, bool FAST>
+__device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, const float4& bi,
+                                    bool i_bnd, const ForceConsts& K, const Src<FAST, 2>& src) {
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  const float Pi = bi.w, rhoi = ai.w;
+  const float react = K.m * ai.w;  // boundary i: m (-rho0 psi_i)
+  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
+    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
+    float4 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], b[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float4 aj = a[k], bj = b[k];
+      // separations in position units; q = r q_unit, grad_i W = (x_j - x_i) gf with
+      // gf = -dW/dr / r(units), and mu = (h/unit) v.r / (r^2 + eps h^2/unit^2)
+      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * rinv;
+      const float gf = -cubic_dwdr(r * K.qu, K.aoh) * rinv;
+      // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
+      // with one reciprocal of d1 * rho_bar; zero for separating pairs.
+      const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+      const float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
+      const float d1 = r2 + K.eh2;
+      const float d2 = 0.5f * (rhoi + aj.w);
+      const float t = K.h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
+      float pi = t * (K.vac + K.vb * (t * d2));
+      pi = (vr < 0.0f) ? pi : 0.0f;
+      const float sf_fluid = gf * (-K.m * ((Pi + bj.w) + pi));
+      const float sf_bnd = gf * (aj.w * Pi);    // gw * (-(rho0 psi_j) P_i)
+      const float sf_react = gf * (react * bj.w);  // -m * gw_b * (-(rho0 psi_i) P_j), gw_b = -gw
+      float sfac = i_bnd ? sf_react : ((aj.w > 0.0f) ? sf_fluid : sf_bnd);
+      sfac = (r2 < K.r2s && r2 > 0.0f) ? sfac : 0.0f;
+      ax += rx * sfac;
+      ay += ry * sfac;
+      az += rz * sfac;
+    }
+  });
+  return make_float3(ax, ay, az);
+}
+
+template <int LPP>
+__global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __restrict__ ctl, Ctl* ctlw,
+                                                      const SceneDev* __restrict__ sc,
+                                                      const SceneState* __restrict__ st, State cur, int integrate,
+                                                      int with_boundary) {
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  const uint32_t CAP = ta.lds_cap;
+  float4* LA = lds;
+  float4* LB = lds + CAP + 1;
+  sentinel_lds(LA, LB, CAP, 2);
+  const double dt = ctl->dt;
+  const float dtf = (float)dt;
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    if (fast) stage_halo<2>(Tb, H, LA, LB, cur.A2, cur.B2);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    ForceConsts K;
+    K.r2s = S.r2s_u;
+    K.qu = S.q_unit;
+    K.aoh = S.alpha * S.inv_h;
+    K.m = S.mass;
+    K.h = S.h_u;
+    K.vb = S.visc_beta;
+    K.eh2 = S.eh2_u;
+    K.vac = -S.visc_alpha * (float)st[Tb.scene].c;
+    const Src<true, 2> fsrc{LA, LB, CAP, cur.A2, cur.B2, &Tb, ta.N};
+    const Src<false, 2> gsrc{LA, LB, CAP, cur.A2, cur.B2, &Tb, ta.N};
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      const uint32_t p = base + threadIdx.x / LPP;
+      const bool act = p < n_own;
+      Own o{0, 0, 0};
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
+      float3 f = make_float3(0.0f, 0.0f, 0.0f);
+      bool i_bnd = false;
+      if (act) {
+        o = own_of(Tb, p);
+        ai = fast ? LA[o.local] : cur.A2[o.s];
+        bi = fast ? LB[o.local] : cur.B2[o.s];
+        i_bnd = is_boundary_w(ai.w);
+        if (!i_bnd || with_boundary) {
+          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
+          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
+          f = fast ? force_walk<LPP, true>(lw, ng, sub, ai, bi, i_bnd, K, fsrc)
+                   : force_walk<LPP, false>(lw, ng, sub, ai, bi, i_bnd, K, gsrc);
+        }
+      }
+      f.x = group_sum<LPP>(f.x);
+      f.y = group_sum<LPP>(f.y);
+      f.z = group_sum<LPP>(f.z);
+      if (act && sub == 0u) {
+        const uint32_t s = o.s;
+        cur.IDF[s] = cur.ID[s];
+        if (i_bnd) {
+          cur.ACC[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          cur.BF[s] = make_float4(f.x, f.y, f.z, 0.0f);
+        } else {
+          const float ax = f.x + S.g[0], ay = f.y + S.g[1], az = f.z + S.g[2];
+          cur.ACC[s] = make_float4(ax, ay, az, 0.0f);
+          cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (integrate) {
+            float vx = bi.x + ax * dtf, vy = bi.y + ay * dtf, vz = bi.z + az * dtf;
+            if (S.dim == 2) vz = 0.0f;
+            const double x = cur.X[s] + (double)vx * dt;
+            const double y = cur.Y[s] + (double)vy * dt;
+            const double z = (S.dim == 2) ? 0.0 : cur.Z[s] + (double)vz * dt;
+            cur.X[s] = x;
+            cur.Y[s] = y;
+            cur.Z[s] = z;
+            cur.A[s] = make_float4(__int_as_float(to_units(x, S.org[0], S.inv_unit)),
+                                   __int_as_float(to_units(y, S.org[1], S.inv_unit)),
+                                   __int_as_float(to_units(z, S.org[2], S.inv_unit)), S.mass);
+            cur.B[s] = make_float4(vx, vy, vz, 0.0f);
+            const float d = cur.DISP[s] + sqrtf(vx * vx + vy * vy + vz * vz) * dtf;
+            cur.DISP[s] = d;
+            if (!(d <= S.skin_half)) ctlw->need_rebuild = 1u;  // also catches NaN
+          }
+        }
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// brick work list: non-empty bricks, compacted at each list build
+// ---------------------------------------------------------------------------
+__global__ void k_brick_flags(uint32_t NB, const Ctl* __restrict__ ctl, int always,
+                              const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ flag) {
+  if (!always && !ctl->rebuilding) return;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < NB) flag[b] = cell_start[(b + 1) * kBrickCells] > cell_start[b * kBrickCells] ? 1u : 0u;
+}
+
+// Halo tables of the non-empty bricks, one wave per brick.
+__global__ void k_brick_tables(const Ctl* __restrict__ ctl, int always, const uint32_t* __restrict__ bricks,
+                               const uint32_t* __restrict__ n_bricks, const SceneDev* __restrict__ sc,
+                               const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ META,
+                               HaloTab* __restrict__ tabs) {
+  if (!always && !ctl->rebuilding) return;
+  const uint32_t nb = *n_bricks;
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) halo_setup(tabs[b], bricks[b], sc, cell_start, META);
+}
+
+__global__ void k_brick_compact(uint32_t NB, const Ctl* __restrict__ ctl, int always,
+                                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                uint32_t* __restrict__ bricks) {
+  if (!always && !ctl->rebuilding) return;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < NB && flag[b]) bricks[pos[b]] = b;
+}
+
+// The list of the particle in `slot`, filtered to r < 2h, as original indices
+// (list order; the host orders them for presentation). One wave.
+__global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, State cur,
+                               const uint16_t* __restrict__ LIST, const uint32_t* __restrict__ CNT, uint32_t cap,
+                               const uint32_t* __restrict__ cell_start, uint64_t* __restrict__ out,
+                               uint32_t* __restrict__ out_len) {
+  __shared__ HaloTab T;
+  const uint32_t gb = cur.CELL[slot] / kBrickCells;
+  halo_setup(T, gb, sc, cell_start, cur.META);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const SceneDev& S = sc[scene_of(cur.META[slot])];
+  const float4 ai = cur.A[slot];
+  const uint16_t* lst = LIST + (uint64_t)slot * cap;
+  const uint32_t n = CNT[slot];
+  uint32_t m = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t e = lst[k];
+    if (e >= T.pre[kHaloCells]) continue;
+    const uint32_t j = halo_slot(T, e);
+    const float4 aj = cur.A[j];
+    const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 < S.r2s_u) out[m++] = cur.ID[j];
+  }
+  *out_len = m;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+#define PEF_LAUNCH_LDS(kernel, grid, block, lds, stream, ...) \
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, __VA_ARGS__)
+
+static inline uint32_t blocks_of(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+static size_t lds_bytes(uint32_t cap, int narr) { return (size_t)(cap + 1) * sizeof(float4) * (size_t)narr; }
+
+template <int LPP>
+static const void* density_fn() { return reinterpret_cast<const void*>(&k_density_tiled<LPP>); }
+template <int LPP>
+static const void* xsph_fn() { return reinterpret_cast<const void*>(&k_xsph_tiled<LPP>); }
+template <int LPP>
+static const void* force_fn() { return reinterpret_cast<const void*>(&k_force_tiled<LPP>); }
+
+static const void* pick(uint32_t lpp, const void* f2, const void* f4, const void* f8, const void* f16) {
+  switch (lpp) {
+    case 2: return f2;
+    case 4: return f4;
+    case 8: return f8;
+    case 16: return f16;
+    default: return nullptr;
+  }
+}
+
+static const void* kernel_fn(int which, uint32_t lpp) {
+  switch (which) {
+    case kTileNlist: return reinterpret_cast<const void*>(&k_nlist_tiled);
+    case kTileDensity:
+      return pick(lpp, density_fn<2>(), density_fn<4>(), density_fn<8>(), density_fn<16>());
+    case kTileXsph: return pick(lpp, xsph_fn<2>(), xsph_fn<4>(), xsph_fn<8>(), xsph_fn<16>());
+    case kTileForce: return pick(lpp, force_fn<2>(), force_fn<4>(), force_fn<8>(), force_fn<16>());
+    default: return nullptr;
+  }
+}
+
+int tiles_configure(TileConfig& c) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  int n_cu = 0, max_lds = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1;
+  if (c.lpp != 2 && c.lpp != 4 && c.lpp != 8 && c.lpp != 16) return 2;
+  if (c.bs < 64 || c.bs > 1024 || c.bs % 64 != 0 || c.nlist_bs < 64 || c.nlist_bs > 1024 || c.nlist_bs % 64 != 0)
+    return 2;
+  c.n_cu = (uint32_t)n_cu;
+  c.max_lds = (uint32_t)max_lds;
+  const size_t table = sizeof(HaloTab) + 256;
+  for (int k = 0; k < kTileKinds; ++k) {
+    const int narr = (k == kTileXsph || k == kTileForce) ? 2 : 1;
+    // one-array kernels are sized so that two workgroups share a CU
+    const size_t budget = (narr == 1 ? (size_t)max_lds / 2 : (size_t)max_lds) - table;
+    uint64_t cap = budget / (sizeof(float4) * narr) - 1;
+    if (c.lds_cap_limit) cap = cap < c.lds_cap_limit ? cap : c.lds_cap_limit;
+    if (cap > kListPad) cap = kListPad;
+    if (cap < 1) cap = 1;
+    c.lds_cap[k] = (uint32_t)cap;
+    const uint32_t bs = (k == kTileNlist) ? c.nlist_bs : c.bs;
+    const void* fn = kernel_fn(k, c.lpp);
+    if (!fn) return 2;
+    const size_t bytes = lds_bytes(c.lds_cap[k], narr);
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 3;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)bs, bytes) != hipSuccess) return 3;
+    if (per_cu < 1) per_cu = 1;
+    c.per_cu[k] = (uint32_t)per_cu;
+    c.grid[k] = (uint32_t)(n_cu * per_cu);
+  }
+  return 0;
+}
+
+void launch_brick_list(hipStream_t st, uint32_t NB, const Ctl* ctl, int always, const uint32_t* cell_start,
+                       uint32_t* flag, uint32_t* pos, uint32_t* block_sums, uint32_t* bricks, const SceneDev* sc,
+                       const uint32_t* META, void* tables) {
+  PEF_LAUNCH_LDS(k_brick_flags, blocks_of(NB, 256), 256, 0, st, NB, ctl, always, cell_start, flag);
+  launch_scan(st, ctl, always, flag, pos, block_sums, NB);
+  PEF_LAUNCH_LDS(k_brick_compact, blocks_of(NB, 256), 256, 0, st, NB, ctl, always, flag, pos, bricks);
+  const uint32_t grid = NB < 4096u ? (NB ? NB : 1u) : 4096u;
+  PEF_LAUNCH_LDS(k_brick_tables, grid, kWave, 0, st, ctl, always, bricks, pos + NB, sc, cell_start, META,
+                 reinterpret_cast<HaloTab*>(tables));
+}
+
+void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
+                        const SceneDev* sc, const State& cur, int with_boundary, int always) {
+  TileArgs a = ta;
+  a.lds_cap = c.lds_cap[kTileNlist];
+  a.queue = ta.queue + 16 * kTileNlist;
+  PEF_LAUNCH_LDS(k_nlist_tiled, c.grid[kTileNlist], c.nlist_bs, lds_bytes(a.lds_cap, 1), st, a, ctl, ctlw, sc, cur,
+                 with_boundary, always);
+}
+
+#define PEF_LPP_SWITCH(lpp, KERNEL, ...)                \
+  switch (lpp) {                                        \
+    case 2: KERNEL(2, __VA_ARGS__); break;              \
+    case 4: KERNEL(4, __VA_ARGS__); break;              \
+    case 8: KERNEL(8, __VA_ARGS__); break;              \
+    default: KERNEL(16, __VA_ARGS__); break;            \
+  }
+
+void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
+                          const State& cur) {
+  TileArgs a = ta;
+  a.lds_cap = c.lds_cap[kTileDensity];
+  a.queue = ta.queue + 16 * kTileDensity;
+  const size_t bytes = lds_bytes(a.lds_cap, 1);
+#define K_DENS(L, ...) PEF_LAUNCH_LDS(k_density_tiled<L>, c.grid[kTileDensity], c.bs, bytes, st, a, sc, cur)
+  PEF_LPP_SWITCH(c.lpp, K_DENS, 0)
+#undef K_DENS
+}
+
+void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
+                       const State& cur) {
+  TileArgs a = ta;
+  a.lds_cap = c.lds_cap[kTileXsph];
+  a.queue = ta.queue + 16 * kTileXsph;
+  const size_t bytes = lds_bytes(a.lds_cap, 2);
+#define K_XSPH(L, ...) PEF_LAUNCH_LDS(k_xsph_tiled<L>, c.grid[kTileXsph], c.bs, bytes, st, a, sc, cur)
+  PEF_LPP_SWITCH(c.lpp, K_XSPH, 0)
+#undef K_XSPH
+}
+
+void launch_force_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
+                        const SceneDev* sc, const SceneState* ss, const State& cur, int integrate,
+                        int with_boundary) {
+  TileArgs a = ta;
+  a.lds_cap = c.lds_cap[kTileForce];
+  a.queue = ta.queue + 16 * kTileForce;
+  const size_t bytes = lds_bytes(a.lds_cap, 2);
+#define K_FORCE(L, ...)                                                                                      \
+  PEF_LAUNCH_LDS(k_force_tiled<L>, c.grid[kTileForce], c.bs, bytes, st, a, ctl, ctlw, sc, ss, cur, integrate, \
+                 with_boundary)
+  PEF_LPP_SWITCH(c.lpp, K_FORCE, 0)
+#undef K_FORCE
+}
+
+void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint16_t* LIST,
+                         const uint32_t* CNT, uint32_t cap, const uint32_t* cell_start, uint64_t* out,
+                         uint32_t* out_len) {
+  PEF_LAUNCH_LDS(k_extract_list, 1, 64, 0, st, slot, sc, cur, LIST, CNT, cap, cell_start, out, out_len);
+}
+
+}  // namespace pef

@@ -11,9 +11,9 @@
 //     k_force (integrate)                 compute_forces + kick + drift
 //     k_decide                            list rebuild needed? (skin criterion)
 //     k_cell_count .. k_copyback          rebuild_grid (when decided)
-//     k_nlist                             neighbour lists (when decided)
-//     k_density                           compute_density
-//     k_xsph                              apply_xsph
+//     k_brick_* + k_nlist_tiled           neighbour lists (when decided)
+//     k_density_tiled                     compute_density
+//     k_xsph_tiled                        apply_xsph
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -25,6 +25,7 @@
 #include "../../include/pef.h"
 #include "pef_common.h"
 #include "pef_launch.h"
+#include "pef_pair.h"
 
 namespace pef {
 
@@ -43,7 +44,7 @@ class World {
   uint32_t margin = 4;
   int with_boundary = 1;
   int use_graph = 1;
-  uint32_t bs = 256;
+  TileConfig tc;
   std::vector<PefParams> params;
   std::vector<SceneDev> scenes;
   std::vector<uint64_t> scene_first, scene_count;
@@ -53,7 +54,13 @@ class World {
   State cur{}, alt{};
   uint32_t *cell_of = nullptr, *rank_of = nullptr, *counts = nullptr, *cell_start = nullptr, *block_sums = nullptr,
            *perm = nullptr;
-  uint32_t *LIST = nullptr, *CNT = nullptr;
+  uint16_t* LIST = nullptr;
+  uint32_t* CNT = nullptr;
+  // brick work list of the tiled kernels
+  uint32_t NB = 0;
+  uint32_t *brick_flag = nullptr, *brick_pos = nullptr, *brick_sums = nullptr, *bricks = nullptr,
+           *queues = nullptr;
+  uint4* brick_tabs = nullptr;
   double* stage = nullptr;  // 3N doubles for getters
   std::vector<void*> allocs;
   double t = 0.0;
@@ -224,9 +231,18 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     if (opt->grid_margin_cells) w.margin = opt->grid_margin_cells;
     if (opt->boundary_force < 0) w.with_boundary = 0;
     if (opt->use_graph < 0) w.use_graph = 0;
-    if (opt->block_size) w.bs = opt->block_size;
+    if (opt->block_size) w.tc.bs = opt->block_size;
+    if (opt->lanes_per_particle) w.tc.lpp = opt->lanes_per_particle;
+    w.tc.lds_cap_limit = opt->lds_halo_capacity;
   }
-  if (!(w.skin >= 0.0 && w.skin <= 1.0) || w.bs % 64 != 0 || w.bs > 1024 || w.margin > 1024) return fail(PEF_ERR_ARG);
+  if (!(w.skin >= 0.0 && w.skin <= 1.0) || w.tc.bs % 64 != 0 || w.tc.bs > 1024 || w.tc.bs < 64 || w.margin > 1024)
+    return fail(PEF_ERR_ARG);
+  if (w.tc.bs % w.tc.lpp != 0) return fail(PEF_ERR_ARG);
+  {
+    const int rc = pef::tiles_configure(w.tc);
+    if (rc == 2) return fail(PEF_ERR_ARG);
+    if (rc != 0) return fail(PEF_ERR_DEVICE);
+  }
   if (!(opt && opt->neighbour_capacity)) {
     // Default capacity: 1.3x the particles a rest-density lattice puts inside
     // the list radius 2h (1 + skin), plus 32, rounded up to a multiple of 32.
@@ -280,7 +296,11 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
       dims[a] = flat ? 1 : (mx[a] - mn[a] + 1 + 2 * m);
       if (lo[a] < INT32_MIN + 2 || lo[a] + dims[a] > INT32_MAX - 2) return fail(PEF_ERR_BOUNDS);
     }
-    uint64_t sc_cells = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2];
+    // brick-major cell ids: bricks of 2 x 2 x 1 cells (the grid is padded to whole bricks)
+    S.bricked = 1;
+    S.nbx = (int32_t)((dims[0] + 1) / 2);
+    S.nby = (int32_t)((dims[1] + 1) / 2);
+    uint64_t sc_cells = (uint64_t)S.nbx * (uint64_t)S.nby * (uint64_t)dims[2] * pef::kBrickCells;
     if (sc_cells > (1ull << 30)) return fail(PEF_ERR_BOUNDS);
     S.ix0 = (int32_t)lo[0];
     S.iy0 = (int32_t)lo[1];
@@ -349,6 +369,13 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   ALLOC(w.block_sums, (size_t)(w.C / pef::kScanTileHost) + 2);
   ALLOC(w.CNT, N);
   ALLOC(w.LIST, (size_t)chunks * 64 * w.cap);
+  w.NB = w.C / pef::kBrickCells;
+  ALLOC(w.brick_flag, (size_t)w.NB);
+  ALLOC(w.brick_pos, (size_t)w.NB + 1);
+  ALLOC(w.brick_sums, (size_t)(w.NB / pef::kScanTileHost) + 2);
+  ALLOC(w.bricks, (size_t)w.NB);
+  ALLOC(w.brick_tabs, (size_t)w.NB * (pef::kHaloTabBytes / 16));
+  ALLOC(w.queues, 16 * pef::kTileKinds);
   ALLOC(w.stage, 3ull * N);
 #undef ALLOC
   if (e != hipSuccess) {
@@ -402,6 +429,8 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   if (e == hipSuccess) e = hipMemcpyAsync(w.ctl_d, &hctl, sizeof(hctl), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(w.counts, 0, (size_t)w.C * 4, st);
   if (e == hipSuccess) e = hipMemsetAsync(w.CNT, 0, (size_t)N * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(w.queues, 0, 16 * pef::kTileKinds * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(w.brick_pos, 0, ((size_t)w.NB + 1) * 4, st);
   if (e == hipSuccess) {
     pef::launch_stage(st, N, w.sc_d, w.cur, dx, dv, dk, dpsi, dn, dscene);
     pef::launch_sentinel(st, N, w.cur);
@@ -424,6 +453,28 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   w.h_kind = std::move(hk);
   *out = pw;
   return PEF_OK;
+}
+
+pef::TileArgs tile_args(World& w) {
+  pef::TileArgs a;
+  a.cell_start = w.cell_start;
+  a.bricks = w.bricks;
+  a.n_bricks = w.brick_pos + w.NB;
+  a.tables = w.brick_tabs;
+  a.queue = w.queues;
+  a.LIST = w.LIST;
+  a.CNT = w.CNT;
+  a.cap = w.cap;
+  a.lds_cap = 0;
+  a.N = w.N;
+  return a;
+}
+
+void launch_density(World& w) { pef::launch_density_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.cur); }
+void launch_xsph(World& w) { pef::launch_xsph_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.cur); }
+void launch_force(World& w, int integrate) {
+  pef::launch_force_tiled(w.stream, w.tc, tile_args(w), w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, integrate,
+                          w.with_boundary);
 }
 
 pef::GridArgs grid_args(World& w) {
@@ -486,8 +537,9 @@ void enqueue_rebuild(World& w, Prof* pr) {
   if (pr) pr->mark("grid_count_scan_scatter");
   pef::launch_finalize(w.stream, w.N, w.ctl_d, w.sc_d, w.cell_of, w.cell_start, w.perm, w.cur, w.alt, w.counts, w.C);
   if (pr) pr->mark("grid_finalize");
-  pef::launch_nlist(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.cell_start, w.cur, w.LIST, w.CNT, w.cap,
-                    w.with_boundary, 0, w.bs);
+  pef::launch_brick_list(w.stream, w.NB, w.ctl_d, 0, w.cell_start, w.brick_flag, w.brick_pos, w.brick_sums,
+                         w.bricks, w.sc_d, w.cur.META, w.brick_tabs);
+  pef::launch_nlist_tiled(w.stream, w.tc, tile_args(w), w.ctl_d, w.ctl_d, w.sc_d, w.cur, w.with_boundary, 0);
   if (pr) pr->mark("nlist");
 }
 
@@ -495,13 +547,12 @@ void enqueue_body(World& w, Prof* pr) {
   if (pr) pr->mark("start");
   pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
   if (pr) pr->mark("pressure");
-  pef::launch_force(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, w.LIST, w.CNT, w.cap, 1, w.with_boundary,
-                    w.bs);
+  launch_force(w, 1);
   if (pr) pr->mark("force");
   enqueue_rebuild(w, pr);
-  pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  launch_density(w);
   if (pr) pr->mark("density");
-  pef::launch_xsph(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  launch_xsph(w);
   if (pr) pr->mark("xsph");
 }
 
@@ -529,7 +580,7 @@ int32_t ensure_density(World& w) {
   Prof pr(w);
   pr.mark("start");
   enqueue_rebuild(w, &pr);
-  pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+  launch_density(w);
   pr.mark("density");
   HIPTRY(hipGetLastError());
   pr.finish();
@@ -584,20 +635,19 @@ int32_t do_phase(World& w, int32_t phase) {
         if (rc) return rc;
         enqueue_rebuild(w, nullptr);
       }
-      pef::launch_density(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+      launch_density(w);
       w.density_valid = true;
       break;
     case PEF_PHASE_PRESSURE:
       pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
       break;
     case PEF_PHASE_FORCES:
-      pef::launch_force(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, w.LIST, w.CNT, w.cap, 0,
-                        w.with_boundary, w.bs);
+      launch_force(w, 0);
       break;
     case PEF_PHASE_XSPH: {
       // XSPH reads B: make it the current velocity first
       HIPTRY(hipMemcpyAsync(w.cur.B, w.cur.B2, (size_t)w.N * sizeof(float4), hipMemcpyDeviceToDevice, w.stream));
-      pef::launch_xsph(w.stream, w.N, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.bs);
+      launch_xsph(w);
       break;
     }
     default:
@@ -633,6 +683,7 @@ int32_t build_probe(World& w) {
   for (uint32_t k = 0; k < w.S; ++k) {
     psc[k].cell = 2.0 * w.params[k].h;  // WcsphWorld::new: UniformGrid::new(support_radius) (wcsph.rs:84-85)
     psc[k].inv_cell = 1.0 / psc[k].cell;
+    psc[k].bricked = 0;  // reference cell order: (ix, iy, iz) lexicographic
   }
   HIPTRY(hipMalloc(&w.probe_sc, w.S * sizeof(pef::SceneDev)));
   HIPTRY(hipMemcpyAsync(w.probe_sc, psc.data(), w.S * sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
@@ -769,7 +820,7 @@ int32_t get_neighbour_list(World& w, uint64_t i, uint64_t* buf, uint64_t cap, ui
   if (e == hipSuccess) e = hipMemcpyAsync(&slot, d_slot + i, 4, hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
   if (e == hipSuccess) {
-    pef::launch_extract_list(w.stream, slot, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, d_out, d_len);
+    pef::launch_extract_list(w.stream, slot, w.sc_d, w.cur, w.LIST, w.CNT, w.cap, w.cell_start, d_out, d_len);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&n, d_len, 4, hipMemcpyDeviceToHost, w.stream);
@@ -1136,7 +1187,7 @@ extern "C" int32_t pefx_debug_dump(PefWorld* world, int32_t which, void* out, ui
     case 5: src = w.cur.META; n = N * 4; break;
     case 6: src = w.cur.CELL; n = N * 4; break;
     case 7: src = w.CNT; n = N * 4; break;
-    case 8: src = w.LIST; n = ((N + 63) / 64) * 64ull * w.cap * 4; break;
+    case 8: src = w.LIST; n = ((N + 63) / 64) * 64ull * w.cap * 2; break;
     case 9: src = w.cell_start; n = ((uint64_t)w.C + 1) * 4; break;
     case 10: src = w.cur.X; n = N * 8; break;
     case 11: src = w.cur.Y; n = N * 8; break;
@@ -1147,6 +1198,37 @@ extern "C" int32_t pefx_debug_dump(PefWorld* world, int32_t which, void* out, ui
   if (!out || bytes < n) return PEF_ERR_BOUNDS;
   if (hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, w.stream) != hipSuccess) return PEF_ERR_PANIC;
   if (hipStreamSynchronize(w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  return PEF_OK;
+}
+
+// Internal measurement aid: launch configuration of the tiled kernels.
+// out[0..21]: bs, nlist_bs, lpp, n_cu, max_lds, lds_cap[4], per_cu[4], grid[4],
+// list capacity, bricks in the grid, non-empty bricks, bricks over the LDS
+// capacity at the last list build, largest list length seen.
+extern "C" int32_t pefx_tile_info(PefWorld* world, uint32_t* out, uint32_t n) {
+  if (!world || !out) return PEF_ERR_NULL;
+  if (n < 23) return PEF_ERR_BOUNDS;
+  World& w = world->w;
+  const pef::TileConfig& c = w.tc;
+  uint32_t k = 0;
+  out[k++] = c.bs;
+  out[k++] = c.nlist_bs;
+  out[k++] = c.lpp;
+  out[k++] = c.n_cu;
+  out[k++] = c.max_lds;
+  for (int i = 0; i < pef::kTileKinds; ++i) out[k++] = c.lds_cap[i];
+  for (int i = 0; i < pef::kTileKinds; ++i) out[k++] = c.per_cu[i];
+  for (int i = 0; i < pef::kTileKinds; ++i) out[k++] = c.grid[i];
+  out[k++] = w.cap;
+  out[k++] = w.NB;
+  uint32_t nb = 0;
+  pef::Ctl h;
+  if (hipMemcpyAsync(&nb, w.brick_pos + w.NB, 4, hipMemcpyDeviceToHost, w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  if (hipMemcpyAsync(&h, w.ctl_d, sizeof(h), hipMemcpyDeviceToHost, w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  if (hipStreamSynchronize(w.stream) != hipSuccess) return PEF_ERR_PANIC;
+  out[k++] = nb;
+  out[k++] = h.slow_bricks;
+  out[k++] = h.max_count;
   return PEF_OK;
 }
 

@@ -78,6 +78,8 @@ class PefOptions(C.Structure):
         ("boundary_force", C.c_int32),
         ("use_graph", C.c_int32),
         ("block_size", C.c_uint32),
+        ("lanes_per_particle", C.c_uint32),
+        ("lds_halo_capacity", C.c_uint32),
     ]
 
 
@@ -300,6 +302,8 @@ class Options:
     boundary_force: bool = True
     use_graph: bool = True
     block_size: int = 0
+    lanes_per_particle: int = 0  # 0 = library default
+    lds_halo_capacity: int = 0  # 0 = as many as LDS holds
 
     def to_c(self) -> PefOptions:
         o = PefOptions()
@@ -309,6 +313,8 @@ class Options:
         o.boundary_force = 1 if self.boundary_force else -1
         o.use_graph = 1 if self.use_graph else -1
         o.block_size = self.block_size
+        o.lanes_per_particle = self.lanes_per_particle
+        o.lds_halo_capacity = self.lds_halo_capacity
         return o
 
 
@@ -488,6 +494,20 @@ class WcsphWorld:
         m, r = C.c_double(), C.c_uint64()
         _check(lib().pef_world_list_stats(self._h, C.byref(m), C.byref(r)), "list_stats")
         return m.value, r.value
+
+    def tile_info(self) -> dict:
+        """Launch configuration of the brick-tiled kernels and brick counts."""
+        out = (C.c_uint32 * 23)()
+        f = lib().pefx_tile_info
+        f.restype = C.c_int32
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32]
+        _check(f(self._h, out, 23), "tile_info")
+        v = list(out)
+        kinds = ("nlist", "density", "xsph", "force")
+        return {"block_size": v[0], "nlist_block_size": v[1], "lanes_per_particle": v[2], "n_cu": v[3],
+                "max_lds": v[4], "lds_cap": dict(zip(kinds, v[5:9])), "per_cu": dict(zip(kinds, v[9:13])),
+                "grid": dict(zip(kinds, v[13:17])), "list_capacity": v[17], "bricks": v[18],
+                "nonempty_bricks": v[19], "slow_bricks": v[20], "max_list": v[21]}
 
 
 def version() -> str:
