@@ -31,7 +31,7 @@ struct TileArgs {
 struct TileConfig {
   uint32_t bs = 1024;       // threads per workgroup, pair kernels
   uint32_t nlist_bs = 512;  // threads per workgroup, list build
-  uint32_t lpp = 8;         // lanes per particle, pair kernels (2, 4, 8, 16)
+  uint32_t lpp = 4;         // lanes per particle, pair kernels (2, 4, 8, 16)
   uint32_t lds_cap_limit = 0;  // optional upper bound on every kernel's LDS halo capacity
   uint32_t n_cu = 0, max_lds = 0;
   uint32_t lds_cap[kTileKinds] = {};
@@ -48,7 +48,7 @@ void launch_brick_list(hipStream_t st, uint32_t NB, const Ctl* ctl, int always, 
 void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
                         const SceneDev* sc, const State& cur, int with_boundary, int always);
 void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
-                          const State& cur);
+                          const SceneState* ss, const State& cur);
 void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
                        const State& cur);
 void launch_force_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,

@@ -55,7 +55,8 @@ struct __align__(16) HaloTab {
   uint32_t gb;                   // global brick index, kNoBrick when the queues are empty
   uint32_t scene;
   uint32_t s0, s1;               // the brick's slot range
-  uint32_t pad[7];
+  int32_t org[3];                // brick centre in position units
+  uint32_t pad[4];
 };
 static_assert(sizeof(HaloTab) == kHaloTabBytes, "halo table size");
 constexpr uint32_t kTabQuads = sizeof(HaloTab) / 16;
@@ -79,6 +80,20 @@ __device__ inline uint32_t halo_cell_of(const HaloTab& T, uint32_t t) {
 __device__ inline uint32_t halo_slot(const HaloTab& T, uint32_t t) {
   const uint32_t k = halo_cell_of(T, t);
   return T.start[k] + (t - T.pre[k]);
+}
+
+// Position relative to the brick centre as FP32. Position units are sized so
+// that every halo position is within 2^24 units of the centre, so the
+// conversion is exact and pair separations are exact differences.
+struct Org {
+  int32_t x, y, z;
+};
+__device__ inline Org org_of(const HaloTab& T) { return Org{T.org[0], T.org[1], T.org[2]}; }
+__device__ inline float rel1(float a, int32_t o) {
+  return (float)(int32_t)((uint32_t)__float_as_int(a) - (uint32_t)o);  // wraps only for the far sentinel
+}
+__device__ inline float4 rel(const float4& a, const Org& o) {
+  return make_float4(rel1(a.x, o.x), rel1(a.y, o.y), rel1(a.z, o.z), a.w);
 }
 
 // One wave (all 64 lanes, lane = threadIdx.x) fills the halo table of global
@@ -119,6 +134,10 @@ __device__ inline void halo_setup(HaloTab& T, uint32_t gb, const SceneDev* __res
     T.scene = scene;
     T.s0 = s0;
     T.s1 = s1;
+    // brick centre: cells [2bx, 2bx+2) x [2by, 2by+2) x [bz, bz+1) of the grid
+    T.org[0] = to_units(((double)S.ix0 + 2.0 * bx + 1.0) * S.cell, S.org[0], S.inv_unit);
+    T.org[1] = to_units(((double)S.iy0 + 2.0 * by + 1.0) * S.cell, S.org[1], S.inv_unit);
+    T.org[2] = (S.dim == 2) ? 0 : to_units(((double)S.iz0 + bz + 0.5) * S.cell, S.org[2], S.inv_unit);
   }
 }
 
@@ -153,7 +172,8 @@ __device__ inline void brick_loop(const TileArgs& ta, HaloTab& T, Work work) {
       const uint32_t cur = __shfl(next, 0, kWave);
       if (cur != kNoBrick) {
         if (threadIdx.x == 0) next = queue_take(ta.queue, nb, x0, tries);
-        if (threadIdx.x < kTabQuads) reinterpret_cast<uint4*>(&T)[threadIdx.x] = tabs[(uint64_t)cur * kTabQuads + threadIdx.x];
+        if (threadIdx.x < kTabQuads)
+          reinterpret_cast<uint4*>(&T)[threadIdx.x] = tabs[(uint64_t)cur * kTabQuads + threadIdx.x];
       } else if (threadIdx.x == 0) {
         T.gb = kNoBrick;
       }
@@ -172,13 +192,16 @@ __device__ inline void brick_loop(const TileArgs& ta, HaloTab& T, Work work) {
   }
 }
 
-// Copies the halo's particles into LDS (one or two arrays). Each thread issues
-// the loads of up to kStageUnroll particles before it writes any of them, so
-// a brick change costs about one memory latency rather than one per particle.
+// LDS records: NARR arrays of float4 per halo particle, array k at R + k *
+// (cap + 1); the first holds the brick-relative position. Separate arrays keep
+// the 16-byte records of random partners spread over all LDS bank groups.
+// Staging issues the loads of up to kStageUnroll particles per thread before
+// it writes any of them, so a brick change costs about one memory latency.
 constexpr uint32_t kStageUnroll = 8;
 template <int NARR>
-__device__ inline void stage_halo(const HaloTab& T, uint32_t H, float4* __restrict__ LA, float4* __restrict__ LB,
+__device__ inline void stage_halo(const HaloTab& T, uint32_t H, float4* __restrict__ R, uint32_t cap,
                                   const float4* __restrict__ GA, const float4* __restrict__ GB) {
+  const Org o = org_of(T);
   for (uint32_t t0 = threadIdx.x; t0 < H; t0 += kStageUnroll * blockDim.x) {
     float4 a[kStageUnroll], b[kStageUnroll];
 #pragma unroll
@@ -194,10 +217,20 @@ __device__ inline void stage_halo(const HaloTab& T, uint32_t H, float4* __restri
     for (uint32_t u = 0; u < kStageUnroll; ++u) {
       const uint32_t t = t0 + u * blockDim.x;
       if (t < H) {
-        LA[t] = a[u];
-        if (NARR == 2) LB[t] = b[u];
+        R[t] = rel(a[u], o);
+        if (NARR == 2) R[cap + 1 + t] = b[u];
       }
     }
+  }
+}
+
+// The sentinel record (index cap): far from every particle, zero weight.
+template <int NARR>
+__device__ inline void sentinel_lds(float4* R, uint32_t cap) {
+  if (threadIdx.x == 0) {
+    const float far = (float)kSentinelUnits;
+    R[cap] = make_float4(far, far, far, 0.0f);
+    if (NARR == 2) R[2 * cap + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
 
@@ -230,16 +263,20 @@ __device__ inline uint2 load_word(const uint2* lw, uint32_t g) {
 }
 
 // Walk list words sub, sub + LPP, ... < ng; fn(word) evaluates four entries.
-// The load of the next word is issued before the current one is evaluated.
+// The loads of the next two words are in flight while a word is evaluated.
+// They are unconditional (indices clamped to the last word) so the compiler
+// can wait for the oldest load only, not for every load outstanding.
 template <int LPP, class Fn>
 __device__ inline void walk_words(const uint2* lw, uint32_t ng, uint32_t sub, Fn fn) {
   if (sub >= ng) return;
+  const uint32_t last = ng - 1u;
   uint2 cur = load_word(lw, sub);
+  uint2 nxt = load_word(lw, min(sub + LPP, last));
   for (uint32_t g = sub; g < ng; g += LPP) {
-    const uint32_t gn = g + LPP;
-    const uint2 nxt = (gn < ng) ? load_word(lw, gn) : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    const uint2 nn = load_word(lw, min(g + 2u * LPP, last));
     fn(cur);
     cur = nxt;
+    nxt = nn;
   }
 }
 
@@ -250,36 +287,39 @@ __device__ inline float group_sum(float v) {
   return v;
 }
 
-// Partner state source: LDS (FAST) or global memory through the halo table.
+// Partner state source: LDS records (FAST) or global memory through the halo
+// table, converted the same way as at staging.
 template <bool FAST, int NARR>
 struct Src {
-  const float4* LA;
-  const float4* LB;
-  uint32_t cap;  // LDS sentinel index
+  const float4* R;  // LDS records
+  uint32_t cap;     // LDS sentinel index
   const float4* GA;
   const float4* GB;
   const HaloTab* T;
   uint32_t N;  // global sentinel slot
+  Org o;
   __device__ inline void get(uint32_t e, float4& a, float4& b) const {
     if (FAST) {
       const uint32_t i = e < cap ? e : cap;  // list padding -> sentinel
-      a = LA[i];
-      if (NARR == 2) b = LB[i];
+      a = R[i];
+      if (NARR == 2) b = R[cap + 1 + i];
     } else {
       const uint32_t s = (e < T->pre[kHaloCells]) ? halo_slot(*T, e) : N;  // padding -> sentinel slot
-      a = GA[s];
+      a = rel(GA[s], o);
+      if (NARR == 2) b = GB[s];
+    }
+  }
+  // the brick's own particle with halo-local index `local` and slot s
+  __device__ inline void own(uint32_t local, uint32_t s, float4& a, float4& b) const {
+    if (FAST) {
+      a = R[local];
+      if (NARR == 2) b = R[cap + 1 + local];
+    } else {
+      a = rel(GA[s], o);
       if (NARR == 2) b = GB[s];
     }
   }
 };
-
-__device__ inline void sentinel_lds(float4* LA, float4* LB, uint32_t cap, int narr) {
-  if (threadIdx.x == 0) {
-    const float far = __int_as_float(kSentinelUnits);
-    LA[cap] = make_float4(far, far, far, 0.0f);
-    if (narr == 2) LB[cap] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // neighbour-list build (grid.rs:122-140 candidates, filtered to the list radius)
@@ -316,7 +356,8 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
   const uint32_t meta = cur.META[s];
   const bool i_bnd = (meta & kMetaBoundary) != 0u;
   const bool i_or = (meta & kMetaOriented) != 0u;
-  const float4 ai = FAST ? LA[o.local] : cur.A[s];
+  const Org org = org_of(T);
+  const float4 ai = FAST ? LA[o.local] : rel(cur.A[s], org);
   const float r2l = S.r2l_u;
   const uint32_t hx = (o.q >> 1) + 1u, hy = (o.q & 1u) + 1u;
   uint16_t* out = LIST + (uint64_t)s * cap;
@@ -333,8 +374,8 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
         bool ok = false;
         if (t < t1) {
           float4 aj;
-          if (FAST) aj = LA[t]; else aj = cur.A[row_slot(T, k0, t)];
-          const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+          if (FAST) aj = LA[t]; else aj = rel(cur.A[row_slot(T, k0, t)], org);
+          const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
           const float r2 = rx * rx + ry * ry + rz * rz;
           ok = (r2 < r2l) && t != o.local;
           if (ok) {
@@ -379,13 +420,13 @@ __global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __
   __shared__ HaloTab T;
   float4* LA = lds;
   const uint32_t CAP = ta.lds_cap;
-  sentinel_lds(LA, nullptr, CAP, 1);
+  sentinel_lds<1>(LA, CAP);
   uint32_t mymax = 0;
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
     const bool encodable = H <= kListPad;  // indices < H must stay below the pad value
-    if (fast) stage_halo<1>(Tb, H, LA, nullptr, cur.A, nullptr);
+    if (fast) stage_halo<1>(Tb, H, LA, CAP, cur.A, nullptr);
     if (threadIdx.x == 0) {
       if (!fast) atomicAdd(&ctlw->slow_bricks, 1u);
       if (!encodable) atomicOr(&ctlw->flags, kFlagNeighbours);
@@ -419,7 +460,9 @@ __global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __
 }
 
 // ---------------------------------------------------------------------------
-// density summation with compensated FP32 sums (wcsph.rs:117-154)
+// density summation with compensated FP32 sums (wcsph.rs:117-154); when the
+// scene's sound speed is fixed, the Tait pressure (eos.rs:65-69) follows in
+// the same pass
 // ---------------------------------------------------------------------------
 template <int LPP, bool FAST>
 __device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, float r2s,
@@ -433,7 +476,7 @@ __device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub,
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float4 aj = a[k];
-      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
       const float w = fabsf(aj.w) * cubic_w(r * qu, 1.0f);  // weight m (fluid) or rho0 psi (boundary)
@@ -448,32 +491,36 @@ __device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub,
 }
 
 template <int LPP>
-__global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const SceneDev* __restrict__ sc, State cur) {
+__global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
+                                                        const SceneState* __restrict__ st, State cur) {
   extern __shared__ float4 lds[];
   __shared__ HaloTab T;
-  float4* LA = lds;
+  float4* R = lds;
   const uint32_t CAP = ta.lds_cap;
-  sentinel_lds(LA, nullptr, CAP, 1);
+  sentinel_lds<1>(R, CAP);
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
-    if (fast) stage_halo<1>(Tb, H, LA, nullptr, cur.A, nullptr);
+    if (fast) stage_halo<1>(Tb, H, R, CAP, cur.A, nullptr);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
     const float r2s = S.r2s_u, qu = S.q_unit;
-    const Src<true, 1> fsrc{LA, nullptr, CAP, cur.A, nullptr, &Tb, ta.N};
-    const Src<false, 1> gsrc{LA, nullptr, CAP, cur.A, nullptr, &Tb, ta.N};
+    const bool eos = S.adapt == 0;  // fixed sound speed: the pressure needs no global reduction first
+    const float Bf = (float)(S.rho0_d * st[Tb.scene].c * st[Tb.scene].c / S.gamma_d);
+    const Org org = org_of(Tb);
+    const Src<true, 1> fsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
+    const Src<false, 1> gsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
       const uint32_t p = base + threadIdx.x / LPP;
       const bool act = p < n_own;
       Own o{0, 0, 0};
-      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), unused;
       float part = 0.0f;
       if (act) {
         o = own_of(Tb, p);
-        ai = fast ? LA[o.local] : cur.A[o.s];
+        if (fast) fsrc.own(o.local, o.s, ai, unused); else gsrc.own(o.local, o.s, ai, unused);
         if (!is_boundary_w(ai.w)) {
           const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
           const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
@@ -483,11 +530,13 @@ __global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const Scene
       }
       const float tot = group_sum<LPP>(part);
       if (act && sub == 0u) {
+        const float4 au = cur.A[o.s];  // position units
         if (is_boundary_w(ai.w)) {
-          cur.A2[o.s] = ai;  // boundary keeps w = -rho0 psi; its density is rho0 (wcsph.rs:126-129)
+          cur.A2[o.s] = au;  // boundary keeps w = -rho0 psi; its density is rho0 (wcsph.rs:126-129)
         } else {
           const float rho = fmaxf(S.alpha * (S.mass + tot), 1e-12f);  // self term m W(0) = m alpha
-          cur.A2[o.s] = make_float4(ai.x, ai.y, ai.z, rho);
+          cur.A2[o.s] = make_float4(au.x, au.y, au.z, rho);
+          if (eos) reinterpret_cast<float*>(cur.B2)[4ull * o.s + 3] = tait_p(rho, S, Bf) / (rho * rho);
         }
       }
     }
@@ -509,7 +558,7 @@ __device__ inline float3 xsph_walk(const uint2* lw, uint32_t ng, uint32_t sub, c
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float4 aj = a[k], bj = b[k];
-      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
       const float wij = cubic_w(r * qu, alpha);
@@ -529,18 +578,18 @@ __global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev
   extern __shared__ float4 lds[];
   __shared__ HaloTab T;
   const uint32_t CAP = ta.lds_cap;
-  float4* LA = lds;
-  float4* LB = lds + CAP + 1;
-  sentinel_lds(LA, LB, CAP, 2);
+  float4* R = lds;
+  sentinel_lds<2>(R, CAP);
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
-    if (fast) stage_halo<2>(Tb, H, LA, LB, cur.A2, cur.B);
+    if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
     const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha, two_m = 2.0f * S.mass, eps = S.xsph_eps;
-    const Src<true, 2> fsrc{LA, LB, CAP, cur.A2, cur.B, &Tb, ta.N};
-    const Src<false, 2> gsrc{LA, LB, CAP, cur.A2, cur.B, &Tb, ta.N};
+    const Org org = org_of(Tb);
+    const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
+    const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
@@ -552,8 +601,7 @@ __global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev
       bool smooth = false;
       if (act) {
         o = own_of(Tb, p);
-        ai = fast ? LA[o.local] : cur.A2[o.s];
-        bi = fast ? LB[o.local] : cur.B[o.s];
+        if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         smooth = !is_boundary_w(ai.w) && eps != 0.0f;
         if (smooth) {
           const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
@@ -584,16 +632,16 @@ __global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev
 // pressure + viscosity + Akinci boundary forces, gravity, kick and drift
 // (wcsph.rs:181-259, 331-342). Gather form: a fluid particle sums its own pair
 // terms; a boundary particle sums the reaction -m f of the fluid particles
-// that see it (wcsph.rs:253-254). Both run the same pair body, so a wave that
-// holds both kinds does not diverge.
+// that see it (wcsph.rs:253-254). The walk is compiled separately for the two
+// kinds, so the pair body has no per-pair branch on the particle's kind.
 // ---------------------------------------------------------------------------
 struct ForceConsts {
   float r2s, qu, aoh, m, h, vb, eh2, vac;
 };
 
-template <int LPP, bool FAST>
+template <int LPP, bool FAST, bool IBND>
 __device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, const float4& bi,
-                                    bool i_bnd, const ForceConsts& K, const Src<FAST, 2>& src) {
+                                    const ForceConsts& K, const Src<FAST, 2>& src) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float Pi = bi.w, rhoi = ai.w;
   const float react = K.m * ai.w;  // boundary i: m (-rho0 psi_i)
@@ -607,24 +655,28 @@ __device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, 
       const float4 aj = a[k], bj = b[k];
       // separations in position units; q = r q_unit, grad_i W = (x_j - x_i) gf with
       // gf = -dW/dr / r(units), and mu = (h/unit) v.r / (r^2 + eps h^2/unit^2)
-      const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float rinv = __builtin_amdgcn_rsqf(r2);
       const float r = r2 * rinv;
       const float gf = -cubic_dwdr(r * K.qu, K.aoh) * rinv;
-      // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
-      // with one reciprocal of d1 * rho_bar; zero for separating pairs.
-      const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
-      const float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
-      const float d1 = r2 + K.eh2;
-      const float d2 = 0.5f * (rhoi + aj.w);
-      const float t = K.h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
-      float pi = t * (K.vac + K.vb * (t * d2));
-      pi = (vr < 0.0f) ? pi : 0.0f;
-      const float sf_fluid = gf * (-K.m * ((Pi + bj.w) + pi));
-      const float sf_bnd = gf * (aj.w * Pi);    // gw * (-(rho0 psi_j) P_i)
-      const float sf_react = gf * (react * bj.w);  // -m * gw_b * (-(rho0 psi_i) P_j), gw_b = -gw
-      float sfac = i_bnd ? sf_react : ((aj.w > 0.0f) ? sf_fluid : sf_bnd);
+      float sfac;
+      if (IBND) {
+        sfac = gf * (react * bj.w);  // -m * gw_b * (-(rho0 psi_i) P_j), gw_b = -gw
+      } else {
+        // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
+        // with one reciprocal of d1 * rho_bar; zero for separating pairs.
+        const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+        const float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
+        const float d1 = r2 + K.eh2;
+        const float d2 = 0.5f * (rhoi + aj.w);
+        const float t = K.h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
+        float pi = t * (K.vac + K.vb * (t * d2));
+        pi = (vr < 0.0f) ? pi : 0.0f;
+        const float sf_fluid = gf * (-K.m * ((Pi + bj.w) + pi));
+        const float sf_bnd = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
+        sfac = (aj.w > 0.0f) ? sf_fluid : sf_bnd;
+      }
       sfac = (r2 < K.r2s && r2 > 0.0f) ? sfac : 0.0f;
       ax += rx * sfac;
       ay += ry * sfac;
@@ -632,6 +684,13 @@ __device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, 
     }
   });
   return make_float3(ax, ay, az);
+}
+
+template <int LPP, bool FAST>
+__device__ inline float3 force_any(bool i_bnd, const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai,
+                                   const float4& bi, const ForceConsts& K, const Src<FAST, 2>& src) {
+  return i_bnd ? force_walk<LPP, FAST, true>(lw, ng, sub, ai, bi, K, src)
+               : force_walk<LPP, FAST, false>(lw, ng, sub, ai, bi, K, src);
 }
 
 template <int LPP>
@@ -642,15 +701,14 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
   extern __shared__ float4 lds[];
   __shared__ HaloTab T;
   const uint32_t CAP = ta.lds_cap;
-  float4* LA = lds;
-  float4* LB = lds + CAP + 1;
-  sentinel_lds(LA, LB, CAP, 2);
+  float4* R = lds;
+  sentinel_lds<2>(R, CAP);
   const double dt = ctl->dt;
   const float dtf = (float)dt;
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
-    if (fast) stage_halo<2>(Tb, H, LA, LB, cur.A2, cur.B2);
+    if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B2);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
     ForceConsts K;
@@ -662,8 +720,9 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
     K.vb = S.visc_beta;
     K.eh2 = S.eh2_u;
     K.vac = -S.visc_alpha * (float)st[Tb.scene].c;
-    const Src<true, 2> fsrc{LA, LB, CAP, cur.A2, cur.B2, &Tb, ta.N};
-    const Src<false, 2> gsrc{LA, LB, CAP, cur.A2, cur.B2, &Tb, ta.N};
+    const Org org = org_of(Tb);
+    const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
+    const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
@@ -675,14 +734,13 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
       bool i_bnd = false;
       if (act) {
         o = own_of(Tb, p);
-        ai = fast ? LA[o.local] : cur.A2[o.s];
-        bi = fast ? LB[o.local] : cur.B2[o.s];
+        if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         i_bnd = is_boundary_w(ai.w);
         if (!i_bnd || with_boundary) {
           const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
           const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
-          f = fast ? force_walk<LPP, true>(lw, ng, sub, ai, bi, i_bnd, K, fsrc)
-                   : force_walk<LPP, false>(lw, ng, sub, ai, bi, i_bnd, K, gsrc);
+          f = fast ? force_any<LPP, true>(i_bnd, lw, ng, sub, ai, bi, K, fsrc)
+                   : force_any<LPP, false>(i_bnd, lw, ng, sub, ai, bi, K, gsrc);
         }
       }
       f.x = group_sum<LPP>(f.x);
@@ -722,7 +780,7 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
 }
 
 // ---------------------------------------------------------------------------
-// brick work list: non-empty bricks, compacted at each list build
+// brick work list: non-empty bricks and their halo tables, at each list build
 // ---------------------------------------------------------------------------
 __global__ void k_brick_flags(uint32_t NB, const Ctl* __restrict__ ctl, int always,
                               const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ flag) {
@@ -761,7 +819,8 @@ __global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, S
   __syncthreads();
   if (threadIdx.x != 0) return;
   const SceneDev& S = sc[scene_of(cur.META[slot])];
-  const float4 ai = cur.A[slot];
+  const Org org = org_of(T);
+  const float4 ai = rel(cur.A[slot], org);
   const uint16_t* lst = LIST + (uint64_t)slot * cap;
   const uint32_t n = CNT[slot];
   uint32_t m = 0;
@@ -769,8 +828,8 @@ __global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, S
     const uint32_t e = lst[k];
     if (e >= T.pre[kHaloCells]) continue;
     const uint32_t j = halo_slot(T, e);
-    const float4 aj = cur.A[j];
-    const float rx = units_diff(aj.x, ai.x), ry = units_diff(aj.y, ai.y), rz = units_diff(aj.z, ai.z);
+    const float4 aj = rel(cur.A[j], org);
+    const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 < S.r2s_u) out[m++] = cur.ID[j];
   }
@@ -879,12 +938,12 @@ void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta,
   }
 
 void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
-                          const State& cur) {
+                          const SceneState* ss, const State& cur) {
   TileArgs a = ta;
   a.lds_cap = c.lds_cap[kTileDensity];
   a.queue = ta.queue + 16 * kTileDensity;
   const size_t bytes = lds_bytes(a.lds_cap, 1);
-#define K_DENS(L, ...) PEF_LAUNCH_LDS(k_density_tiled<L>, c.grid[kTileDensity], c.bs, bytes, st, a, sc, cur)
+#define K_DENS(L, ...) PEF_LAUNCH_LDS(k_density_tiled<L>, c.grid[kTileDensity], c.bs, bytes, st, a, sc, ss, cur)
   PEF_LPP_SWITCH(c.lpp, K_DENS, 0)
 #undef K_DENS
 }

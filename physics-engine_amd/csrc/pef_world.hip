@@ -65,6 +65,10 @@ class World {
   std::vector<void*> allocs;
   double t = 0.0;
   bool density_valid = false;
+  // Some scene adapts its sound speed: the pressure needs the global max speed
+  // and runs as its own pass; otherwise the density pass computes it.
+  bool any_adapt = false;
+  bool pressure_stale = false;  // parameters changed since the last pressure evaluation
   // static per-particle inputs in original order (change only through set_state)
   std::vector<double> h_psi, h_nout;
   std::vector<uint8_t> h_kind;
@@ -309,8 +313,11 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     S.ny = (int32_t)dims[1];
     S.nz = (int32_t)dims[2];
     // integer position frame: origin at the grid centre, 2^30 units span the grid
+    // 2^30 units span the grid, and a brick's halo (within 2.25 cells of the
+    // brick centre) spans less than 2^24 units, so halo positions convert to
+    // FP32 exactly relative to the centre.
     const double ext = (double)std::max(dims[0], std::max(dims[1], dims[2])) * S.cell;
-    S.unit_d = ext / 1073741824.0;
+    S.unit_d = std::max(ext / 1073741824.0, 2.25 * S.cell / 16777216.0);
     S.inv_unit = 1.0 / S.unit_d;
     for (int a = 0; a < 3; ++a) S.org[a] = ((double)lo[a] + 0.5 * (double)dims[a]) * S.cell;
     if (p.dim == 2) S.org[2] = 0.0;
@@ -448,6 +455,8 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     return fail(PEF_ERR_PANIC);
   }
   w.density_valid = false;
+  w.any_adapt = false;
+  for (uint32_t k = 0; k < n_scenes; ++k) w.any_adapt = w.any_adapt || params[k].adapt_sound_speed != 0;
   w.h_psi = std::move(hpsi);
   w.h_nout = std::move(hn);
   w.h_kind = std::move(hk);
@@ -470,7 +479,9 @@ pef::TileArgs tile_args(World& w) {
   return a;
 }
 
-void launch_density(World& w) { pef::launch_density_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.cur); }
+void launch_density(World& w) {
+  pef::launch_density_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.st_d, w.cur);
+}
 void launch_xsph(World& w) { pef::launch_xsph_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.cur); }
 void launch_force(World& w, int integrate) {
   pef::launch_force_tiled(w.stream, w.tc, tile_args(w), w.ctl_d, w.ctl_d, w.sc_d, w.st_d, w.cur, integrate,
@@ -545,8 +556,10 @@ void enqueue_rebuild(World& w, Prof* pr) {
 
 void enqueue_body(World& w, Prof* pr) {
   if (pr) pr->mark("start");
-  pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
-  if (pr) pr->mark("pressure");
+  if (w.any_adapt) {
+    pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+    if (pr) pr->mark("pressure");
+  }
   launch_force(w, 1);
   if (pr) pr->mark("force");
   enqueue_rebuild(w, pr);
@@ -596,6 +609,11 @@ int32_t do_step(World& w, double dt, uint32_t nsteps) {
   int32_t rc = ensure_density(w);
   if (rc) return rc;
   w.probe_valid = false;
+  if (w.pressure_stale) {  // new parameters: the step body evaluates no pressure of its own
+    pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+    HIPTRY(hipGetLastError());
+    w.pressure_stale = false;
+  }
   if (w.profiling || !w.use_graph) {
     for (uint32_t k = 0; k < nsteps; ++k) {
       Prof pr(w);
@@ -640,6 +658,7 @@ int32_t do_phase(World& w, int32_t phase) {
       break;
     case PEF_PHASE_PRESSURE:
       pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+      w.pressure_stale = false;
       break;
     case PEF_PHASE_FORCES:
       launch_force(w, 0);
@@ -1086,6 +1105,14 @@ int32_t pef_world_set_params(PefWorld* world, uint32_t scene, const PefParams* p
     s.pad = 0;
     HIPTRY(hipMemcpyAsync(w.st_d + scene, &s, sizeof(s), hipMemcpyHostToDevice, w.stream));
     HIPTRY(hipStreamSynchronize(w.stream));
+    bool any = false;
+    for (const PefParams& q : w.params) any = any || q.adapt_sound_speed != 0;
+    if (any != w.any_adapt && w.graph) {  // the captured step body changes
+      (void)hipGraphExecDestroy(w.graph);
+      w.graph = nullptr;
+    }
+    w.any_adapt = any;
+    w.pressure_stale = !any;
     return (int32_t)PEF_OK;
   });
 }
