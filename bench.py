@@ -23,26 +23,65 @@ for sub in ("physics-engine_amd", "oracle", "tests"):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+DEFAULT_SKIN = 0.04  # the library's default Verlet skin (fraction of 2h)
 METRIC = "particle-updates/s (1M-particle SPH) + surrogate speedup at ≤0.5% density err"
 
 
-def kernel_bytes(n_fluid, n_bnd, list_entries_total):
-    """Algorithmic HBM bytes per launch of the step kernels.
+def kernel_bytes(n_fluid, n_bnd, list_bytes):
+    """Algorithmic HBM bytes per launch of the three pair kernels.
 
-    Own-particle streams only (the per-particle reads and writes each kernel
-    makes once) plus the neighbour-list entries it reads (4 B each); gathers of
-    neighbour state are served from cache and are not counted.
+    Each particle's own reads and writes, counted once, plus the neighbour-list
+    bytes the kernel streams (16-bit entries padded to words of four). Partner
+    state is staged per brick through LDS and, like the neighbour gathers of a
+    cell-sorted layout, is not counted.
     """
-    E = list_entries_total
+    L = list_bytes
     return {
-        # fluid: META A2 B2 CNT ID CELL DISP X Y Z read (76) + list; IDF ACC BF X Y Z A B DISP written (96)
-        # boundary: META A2 B2 CNT ID read (44) + list; IDF ACC BF written (36)
-        "force": 172 * n_fluid + 80 * n_bnd + 4 * E,
-        # fluid: A META CNT read + list, A2 written; boundary: A read, A2 written
-        "density": 40 * n_fluid + 32 * n_bnd + 4 * E,
-        # fluid: A2 B B2 META CNT read + list, B2 written; boundary: A2 B B2 read, B2 written
-        "xsph": 72 * n_fluid + 64 * n_bnd + 4 * E,
+        # fluid: A2 B2 CNT ID X Y Z DISP read (68), IDF ACC BF X Y Z A B DISP written (96)
+        # boundary: A2 B2 CNT ID read (40), IDF ACC BF written (36)
+        "force": 164 * n_fluid + 76 * n_bnd + L,
+        # fluid: A CNT read, A again at the store (36), A2 and P written (20); boundary: A twice, A2 written
+        "density": 56 * n_fluid + 48 * n_bnd + L,
+        # fluid: A2 B CNT B2 read (52), B2 written (16); boundary: A2 B B2 read, B2 written
+        "xsph": 68 * n_fluid + 64 * n_bnd + L,
     }
+
+
+def step_bytes(n, n_fluid, n_cells, k_mean):
+    """Whole-step algorithmic bytes of the plain SPH step in the survey's model:
+    120 N + (196 + 16 K) N_f + 8 N_cells, K = mean neighbours within 2h of a
+    fluid particle (the four pair passes read a 4-byte list entry per pair)."""
+    return 120 * n + (196 + 16 * k_mean) * n_fluid + 8 * n_cells
+
+
+def mean_support_neighbours(x, kind, h, sample=20000, seed=0):
+    """Mean count of particles within 2h (self excluded) of a fixed random
+    sample of fluid particles, from the device positions."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    fl = np.nonzero(kind == 0)[0]
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(fl, size=min(sample, len(fl)), replace=False)
+    tree = cKDTree(x)
+    counts = tree.query_ball_point(x[pick], r=2.0 * h, return_length=True)
+    return float(np.mean(counts) - 1.0)
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    counter's reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().strip("|").split("|")
+        if len(parts) >= 2 and parts[0].strip() == kernel:
+            try:
+                return int(float(parts[-1].strip()))
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(cfg, seconds_hint=10.0):
@@ -56,10 +95,18 @@ def cpu_baseline(cfg, seconds_hint=10.0):
     t0 = time.perf_counter()
     W.step(cfg.dt, 1)
     dt = time.perf_counter() - t0
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": cfg.n_fluid / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
             "sample": f"{cfg.name}: 1 step of the same scene, f64 oracle (reference op order, zero-contribution "
-                      f"candidates skipped, bit-identical to the reference-structured path), 1 thread, "
-                      f"{dt:.2f} s"}
+                      f"candidates skipped, bit-identical to the reference-structured path), 1 thread of "
+                      f"{os.cpu_count()} ({model}), {dt:.2f} s"}
 
 
 def main():
@@ -104,6 +151,7 @@ def main():
 
     mean_entries, rebuilds0 = W.list_stats()
     diag = W.diagnostics()
+    tiles = W.tile_info()
 
     # per-kernel HIP-event durations on the world's stream, same workload
     W.set_profiling(True)
@@ -112,14 +160,23 @@ def main():
     W.set_profiling(False)
     _, rebuilds1 = W.list_stats()
     per_launch_ms = {k: v / args.profile_steps for k, v in times.items()}
-    E = mean_entries * (n_fluid + n_bnd)
-    kb = kernel_bytes(n_fluid, n_bnd, E)
+    # list bytes: 16-bit entries padded to words of four (1.5 pad entries per list on average)
+    n_listed = n_fluid + n_bnd
+    list_bytes = 2.0 * (mean_entries * n_listed + 1.5 * n_listed)
+    kb = kernel_bytes(n_fluid, n_bnd, list_bytes)
     dom = max(("force", "density", "xsph"), key=lambda k: per_launch_ms.get(k, 0.0))
     dom_ms = per_launch_ms[dom]
     achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = profiled_traffic(f"k_{dom}_tiled", cfg.name)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_{dom}",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}_tiled",
             "bytes_per_launch": int(kb[dom]), "avg_launch_ms": round(dom_ms, 5)}
+    k_mean = mean_support_neighbours(W.positions(), parts.kind, cfg.h)
+    n_cells = tiles["bricks"] * 4
+    sb = step_bytes(n_fluid + n_bnd, n_fluid, n_cells, k_mean)
+    step_roof = {"bytes_per_step": int(sb), "mean_neighbours_within_2h": round(k_mean, 1),
+                 "achieved": round(sb / (ms_per_step * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -135,12 +192,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: {n_fluid} fluid + {n_bnd} Akinci wall particles, dx {cfg.dx}, "
                                f"h {cfg.h}, dt {cfg.dt}, tank {cfg.tank}, jitter 0.01 dx seed {cfg.seed}",
-                   "n_fluid": n_fluid, "n_boundary": n_bnd, "skin": args.skin or 0.08,
+                   "n_fluid": n_fluid, "n_boundary": n_bnd, "skin": args.skin or DEFAULT_SKIN,
                    "mean_list_entries": round(mean_entries, 1), "list_rebuilds_in_profile": rebuilds1 - rebuilds0,
                    "surrogate_speedup": None},
         "roofline": roof,
+        "step_roofline": step_roof,
         "kernel_ms": {k: round(v, 5) for k, v in per_launch_ms.items()},
-        "tiles": W.tile_info(),
+        "tiles": tiles,
         "mean_density_error": diag["mean_density_error"],
     }
     if args.calibration_copy_mib:

@@ -95,7 +95,7 @@ typedef struct PefParticles {
 typedef struct PefOptions {
   uint32_t neighbour_capacity; /* list slots per particle (default: 1.3x a rest lattice's count
                                   within 2h (1 + skin), plus 32) */
-  double skin;                 /* Verlet skin as a fraction of 2h (default 0.08) */
+  double skin;                 /* Verlet skin as a fraction of 2h (default 0.04) */
   uint32_t grid_margin_cells;  /* empty cells kept around the initial bounding box (default 4) */
   int32_t boundary_force;      /* compute the Akinci reactions each step (default 1; -1 = off) */
   int32_t use_graph;           /* replay captured HIP graphs for multi-step calls (default 1; -1 = off) */
@@ -199,9 +199,10 @@ int32_t pef_world_get_grid(PefWorld* world, int32_t* keys, uint64_t* order, int3
 int32_t pef_world_get_neighbours(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap,
                                  uint64_t* out_len);
 
-/* The device's own neighbour list of particle i, filtered to r < 2h, in
- * original indices and in the reference's candidate order. The density, force
- * and XSPH kernels sum exactly these pairs. */
+/* The device's own neighbour list of particle i (wcsph.rs:70 neigh_lists,
+ * which holds the unfiltered candidates), filtered to r < 2h: the pairs the
+ * density, force and XSPH kernels sum with a non-zero kernel weight, as
+ * original indices in ascending order. */
 int32_t pef_world_get_neighbour_list(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap,
                                      uint64_t* out_len);
 
