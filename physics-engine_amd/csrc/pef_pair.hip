@@ -248,9 +248,17 @@ __device__ inline Own own_of(const HaloTab& T, uint32_t p) {
 }
 
 // ---------------------------------------------------------------------------
-// list storage: u16 entries, contiguous per slot (LIST[s * cap + k]), read as
-// 8-byte words of four entries
+// list storage: u16 entries, contiguous per slot, read as 8-byte words of four
+// entries. Each block of 16 entries is stored transposed: logical entry
+// 16 b + 4 k + w sits at 16 b + 4 w + k, so word 4 b + w holds entries
+// 16 b + w + {0, 4, 8, 12}. Lanes that walk consecutive words then read
+// consecutive list entries at each step. Lists are in ascending halo order,
+// so those are nearby LDS records in distinct bank groups, and neighbouring
+// particles, whose lists overlap, read the same records. Lists are padded to
+// whole blocks with kListPad.
 // ---------------------------------------------------------------------------
+__device__ inline uint32_t list_pos(uint32_t k) { return (k & ~15u) | ((k & 3u) << 2) | ((k >> 2) & 3u); }
+__device__ inline uint32_t list_words_used(uint32_t cnt) { return ((cnt + 15u) >> 4) << 2; }
 __device__ inline const uint2* list_words(const uint16_t* LIST, uint32_t s, uint32_t cap) {
   return reinterpret_cast<const uint2*>(LIST + (uint64_t)s * cap);
 }
@@ -266,18 +274,63 @@ __device__ inline uint2 load_word(const uint2* lw, uint32_t g) {
 // The loads of the next two words are in flight while a word is evaluated.
 // They are unconditional (indices clamped to the last word) so the compiler
 // can wait for the oldest load only, not for every load outstanding.
+// walk_words2 evaluates two words (eight entries) per step: all eight partner
+// loads are issued before the first pair is evaluated, which hides LDS
+// latency; a missing second word at the end is replaced by padding (the
+// sentinel). walk_words1 takes one word per step, for pair bodies whose
+// registers would not hold eight partners.
 template <int LPP, class Fn>
-__device__ inline void walk_words(const uint2* lw, uint32_t ng, uint32_t sub, Fn fn) {
+__device__ inline void walk_words2(const uint2* lw, uint32_t ng, uint32_t sub, uint2 a, uint2 b, Fn fn) {
   if (sub >= ng) return;
   const uint32_t last = ng - 1u;
-  uint2 cur = load_word(lw, sub);
-  uint2 nxt = load_word(lw, min(sub + LPP, last));
+  const uint2 pad = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  for (uint32_t g = sub; g < ng; g += 2u * LPP) {
+    const uint2 na = load_word(lw, min(g + 2u * LPP, last));
+    const uint2 nb = load_word(lw, min(g + 3u * LPP, last));
+    fn(a, (g + LPP < ng) ? b : pad);
+    a = na;
+    b = nb;
+  }
+}
+template <int LPP, class Fn>
+__device__ inline void walk_words1(const uint2* lw, uint32_t ng, uint32_t sub, uint2 cur, uint2 nxt, Fn fn) {
+  if (sub >= ng) return;
+  const uint32_t last = ng - 1u;
   for (uint32_t g = sub; g < ng; g += LPP) {
     const uint2 nn = load_word(lw, min(g + 2u * LPP, last));
     fn(cur);
     cur = nxt;
     nxt = nn;
   }
+}
+
+// One round of a pair kernel: the brick particle of this lane group, its list
+// count and its first two list words. The first round's are loaded before the
+// halo is staged, so their latency overlaps the staging.
+struct Round {
+  Own o;
+  bool act;
+  uint32_t cnt;
+  uint2 w0, w1;
+  const uint2* lw;
+};
+template <int LPP>
+__device__ inline Round round_load(const HaloTab& T, const TileArgs& ta, uint32_t p, uint32_t n_own, uint32_t sub) {
+  Round r;
+  r.act = p < n_own;
+  r.o = Own{0, 0, 0};
+  r.cnt = 0;
+  r.w0 = r.w1 = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  r.lw = nullptr;
+  if (r.act) {
+    r.o = own_of(T, p);
+    r.lw = list_words(ta.LIST, r.o.s, ta.cap);
+    const uint32_t last = (ta.cap >> 2) - 1u;  // words beyond the count are read but never used
+    r.w0 = load_word(r.lw, min(sub, last));
+    r.w1 = load_word(r.lw, min(sub + LPP, last));
+    r.cnt = ta.CNT[r.o.s];
+  }
+  return r;
 }
 
 template <int LPP>
@@ -400,15 +453,15 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
         const uint64_t m = (__ballot(ok) >> gshift) & kNlistGroupMask;
         if (ok) {
           const uint32_t k = cnt + (uint32_t)__popcll(m & below);
-          if (k < cap) out[k] = (uint16_t)t;
+          if (k < cap) out[list_pos(k)] = (uint16_t)t;
         }
         cnt += (uint32_t)__popcll(m);
       }
     }
   }
   const uint32_t stored = cnt < cap ? cnt : cap;
-  const uint32_t padn = (4u - (stored & 3u)) & 3u;  // pad the last word
-  if (sub < padn) out[stored + sub] = (uint16_t)kListPad;
+  const uint32_t padn = (16u - (stored & 15u)) & 15u;  // pad the last block
+  if (sub < padn) out[list_pos(stored + sub)] = (uint16_t)kListPad;
   return cnt;
 }
 
@@ -465,16 +518,17 @@ __global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __
 // the same pass
 // ---------------------------------------------------------------------------
 template <int LPP, bool FAST>
-__device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, float r2s,
-                                     float qu, const Src<FAST, 1>& src) {
+__device__ inline float density_walk(const Round& r, uint32_t sub, const float4& ai, float r2s, float qu,
+                                     const Src<FAST, 1>& src) {
   float sum = 0.0f, comp = 0.0f;
-  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
-    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
-    float4 a[4], unused;
+  walk_words2<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd, const uint2 wd2) {
+    const uint32_t e[8] = {wd.x & 0xFFFFu,  wd.x >> 16,  wd.y & 0xFFFFu,  wd.y >> 16,
+                           wd2.x & 0xFFFFu, wd2.x >> 16, wd2.y & 0xFFFFu, wd2.y >> 16};
+    float4 a[8], unused;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], unused);
+    for (int k = 0; k < 8; ++k) src.get(e[k], a[k], unused);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
       const float4 aj = a[k];
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
@@ -490,8 +544,10 @@ __device__ inline float density_walk(const uint2* lw, uint32_t ng, uint32_t sub,
   return sum - comp;
 }
 
-template <int LPP>
-__global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
+// MAXT is the largest workgroup the instantiation is launched with: 1024
+// threads leave 128 VGPRs per lane, 512 threads leave 256.
+template <int LPP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
                                                         const SceneState* __restrict__ st, State cur) {
   extern __shared__ float4 lds[];
   __shared__ HaloTab T;
@@ -501,6 +557,9 @@ __global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const Scene
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<1>(Tb, H, R, CAP, cur.A, nullptr);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -510,23 +569,17 @@ __global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const Scene
     const Org org = org_of(Tb);
     const Src<true, 1> fsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
     const Src<false, 1> gsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
-    const uint32_t n_own = Tb.s1 - Tb.s0;
-    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      const uint32_t p = base + threadIdx.x / LPP;
-      const bool act = p < n_own;
-      Own o{0, 0, 0};
+      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      const bool act = rd.act;
+      const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), unused;
       float part = 0.0f;
       if (act) {
-        o = own_of(Tb, p);
         if (fast) fsrc.own(o.local, o.s, ai, unused); else gsrc.own(o.local, o.s, ai, unused);
-        if (!is_boundary_w(ai.w)) {
-          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
-          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
-          part = fast ? density_walk<LPP, true>(lw, ng, sub, ai, r2s, qu, fsrc)
-                      : density_walk<LPP, false>(lw, ng, sub, ai, r2s, qu, gsrc);
-        }
+        if (!is_boundary_w(ai.w))
+          part = fast ? density_walk<LPP, true>(rd, sub, ai, r2s, qu, fsrc)
+                      : density_walk<LPP, false>(rd, sub, ai, r2s, qu, gsrc);
       }
       const float tot = group_sum<LPP>(part);
       if (act && sub == 0u) {
@@ -547,16 +600,17 @@ __global__ void __launch_bounds__(1024) k_density_tiled(TileArgs ta, const Scene
 // XSPH velocity smoothing (wcsph.rs:283-323), Jacobi: reads B, writes B2
 // ---------------------------------------------------------------------------
 template <int LPP, bool FAST>
-__device__ inline float3 xsph_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, const float4& bi,
-                                   float r2s, float qu, float alpha, float two_m, const Src<FAST, 2>& src) {
+__device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi, float r2s,
+                                   float qu, float alpha, float two_m, const Src<FAST, 2>& src) {
   float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
-  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
-    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
-    float4 a[4], b[4];
+  walk_words2<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd, const uint2 wd2) {
+    const uint32_t e[8] = {wd.x & 0xFFFFu,  wd.x >> 16,  wd.y & 0xFFFFu,  wd.y >> 16,
+                           wd2.x & 0xFFFFu, wd2.x >> 16, wd2.y & 0xFFFFu, wd2.y >> 16};
+    float4 a[8], b[8];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], b[k]);
+    for (int k = 0; k < 8; ++k) src.get(e[k], a[k], b[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
       const float4 aj = a[k], bj = b[k];
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
@@ -573,8 +627,8 @@ __device__ inline float3 xsph_walk(const uint2* lw, uint32_t ng, uint32_t sub, c
   return make_float3(dvx, dvy, dvz);
 }
 
-template <int LPP>
-__global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev* __restrict__ sc, State cur) {
+template <int LPP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev* __restrict__ sc, State cur) {
   extern __shared__ float4 lds[];
   __shared__ HaloTab T;
   const uint32_t CAP = ta.lds_cap;
@@ -583,6 +637,9 @@ __global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -590,25 +647,19 @@ __global__ void __launch_bounds__(1024) k_xsph_tiled(TileArgs ta, const SceneDev
     const Org org = org_of(Tb);
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
-    const uint32_t n_own = Tb.s1 - Tb.s0;
-    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      const uint32_t p = base + threadIdx.x / LPP;
-      const bool act = p < n_own;
-      Own o{0, 0, 0};
+      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      const bool act = rd.act;
+      const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
       float3 dv = make_float3(0.0f, 0.0f, 0.0f);
       bool smooth = false;
       if (act) {
-        o = own_of(Tb, p);
         if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         smooth = !is_boundary_w(ai.w) && eps != 0.0f;
-        if (smooth) {
-          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
-          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
-          dv = fast ? xsph_walk<LPP, true>(lw, ng, sub, ai, bi, r2s, qu, alpha, two_m, fsrc)
-                    : xsph_walk<LPP, false>(lw, ng, sub, ai, bi, r2s, qu, alpha, two_m, gsrc);
-        }
+        if (smooth)
+          dv = fast ? xsph_walk<LPP, true>(rd, sub, ai, bi, r2s, qu, alpha, two_m, fsrc)
+                    : xsph_walk<LPP, false>(rd, sub, ai, bi, r2s, qu, alpha, two_m, gsrc);
       }
       dv.x = group_sum<LPP>(dv.x);
       dv.y = group_sum<LPP>(dv.y);
@@ -640,12 +691,12 @@ struct ForceConsts {
 };
 
 template <int LPP, bool FAST, bool IBND>
-__device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai, const float4& bi,
+__device__ inline float3 force_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi,
                                     const ForceConsts& K, const Src<FAST, 2>& src) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float Pi = bi.w, rhoi = ai.w;
   const float react = K.m * ai.w;  // boundary i: m (-rho0 psi_i)
-  walk_words<LPP>(lw, ng, sub, [&](const uint2 wd) {
+  walk_words1<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd) {
     const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
     float4 a[4], b[4];
 #pragma unroll
@@ -687,14 +738,14 @@ __device__ inline float3 force_walk(const uint2* lw, uint32_t ng, uint32_t sub, 
 }
 
 template <int LPP, bool FAST>
-__device__ inline float3 force_any(bool i_bnd, const uint2* lw, uint32_t ng, uint32_t sub, const float4& ai,
-                                   const float4& bi, const ForceConsts& K, const Src<FAST, 2>& src) {
-  return i_bnd ? force_walk<LPP, FAST, true>(lw, ng, sub, ai, bi, K, src)
-               : force_walk<LPP, FAST, false>(lw, ng, sub, ai, bi, K, src);
+__device__ inline float3 force_any(bool i_bnd, const Round& r, uint32_t sub, const float4& ai, const float4& bi,
+                                   const ForceConsts& K, const Src<FAST, 2>& src) {
+  return i_bnd ? force_walk<LPP, FAST, true>(r, sub, ai, bi, K, src)
+               : force_walk<LPP, FAST, false>(r, sub, ai, bi, K, src);
 }
 
-template <int LPP>
-__global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __restrict__ ctl, Ctl* ctlw,
+template <int LPP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_force_tiled(TileArgs ta, const Ctl* __restrict__ ctl, Ctl* ctlw,
                                                       const SceneDev* __restrict__ sc,
                                                       const SceneState* __restrict__ st, State cur, int integrate,
                                                       int with_boundary) {
@@ -708,6 +759,9 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
   brick_loop(ta, T, [&](const HaloTab& Tb) {
     const uint32_t H = Tb.pre[kHaloCells];
     const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
+    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B2);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -723,25 +777,19 @@ __global__ void __launch_bounds__(1024) k_force_tiled(TileArgs ta, const Ctl* __
     const Org org = org_of(Tb);
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
-    const uint32_t n_own = Tb.s1 - Tb.s0;
-    const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      const uint32_t p = base + threadIdx.x / LPP;
-      const bool act = p < n_own;
-      Own o{0, 0, 0};
+      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      const bool act = rd.act;
+      const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
       float3 f = make_float3(0.0f, 0.0f, 0.0f);
       bool i_bnd = false;
       if (act) {
-        o = own_of(Tb, p);
         if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         i_bnd = is_boundary_w(ai.w);
-        if (!i_bnd || with_boundary) {
-          const uint32_t ng = (ta.CNT[o.s] + 3u) >> 2;
-          const uint2* lw = list_words(ta.LIST, o.s, ta.cap);
-          f = fast ? force_any<LPP, true>(i_bnd, lw, ng, sub, ai, bi, K, fsrc)
-                   : force_any<LPP, false>(i_bnd, lw, ng, sub, ai, bi, K, gsrc);
-        }
+        if (!i_bnd || with_boundary)
+          f = fast ? force_any<LPP, true>(i_bnd, rd, sub, ai, bi, K, fsrc)
+                   : force_any<LPP, false>(i_bnd, rd, sub, ai, bi, K, gsrc);
       }
       f.x = group_sum<LPP>(f.x);
       f.y = group_sum<LPP>(f.y);
@@ -825,7 +873,7 @@ __global__ void k_extract_list(uint32_t slot, const SceneDev* __restrict__ sc, S
   const uint32_t n = CNT[slot];
   uint32_t m = 0;
   for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t e = lst[k];
+    const uint32_t e = lst[list_pos(k)];
     if (e >= T.pre[kHaloCells]) continue;
     const uint32_t j = halo_slot(T, e);
     const float4 aj = rel(cur.A[j], org);
@@ -846,30 +894,30 @@ static inline uint32_t blocks_of(uint64_t n, uint32_t bs) { return (uint32_t)((n
 
 static size_t lds_bytes(uint32_t cap, int narr) { return (size_t)(cap + 1) * sizeof(float4) * (size_t)narr; }
 
-template <int LPP>
-static const void* density_fn() { return reinterpret_cast<const void*>(&k_density_tiled<LPP>); }
-template <int LPP>
-static const void* xsph_fn() { return reinterpret_cast<const void*>(&k_xsph_tiled<LPP>); }
-template <int LPP>
-static const void* force_fn() { return reinterpret_cast<const void*>(&k_force_tiled<LPP>); }
+// Instantiations: lanes per particle {2, 4, 8, 16} x largest workgroup {512, 1024}.
+#define PEF_VARIANTS(K)                                                                                   \
+  {reinterpret_cast<const void*>(&K<2, 512>), reinterpret_cast<const void*>(&K<4, 512>),                 \
+   reinterpret_cast<const void*>(&K<8, 512>), reinterpret_cast<const void*>(&K<16, 512>),                \
+   reinterpret_cast<const void*>(&K<2, 1024>), reinterpret_cast<const void*>(&K<4, 1024>),               \
+   reinterpret_cast<const void*>(&K<8, 1024>), reinterpret_cast<const void*>(&K<16, 1024>)}
 
-static const void* pick(uint32_t lpp, const void* f2, const void* f4, const void* f8, const void* f16) {
-  switch (lpp) {
-    case 2: return f2;
-    case 4: return f4;
-    case 8: return f8;
-    case 16: return f16;
-    default: return nullptr;
-  }
+static int variant_of(uint32_t lpp, uint32_t bs) {
+  int l = lpp == 2 ? 0 : lpp == 4 ? 1 : lpp == 8 ? 2 : lpp == 16 ? 3 : -1;
+  if (l < 0) return -1;
+  return l + (bs > 512 ? 4 : 0);
 }
 
-static const void* kernel_fn(int which, uint32_t lpp) {
+static const void* kernel_fn(int which, uint32_t lpp, uint32_t bs) {
+  static const void* dens[8] = PEF_VARIANTS(k_density_tiled);
+  static const void* xsph[8] = PEF_VARIANTS(k_xsph_tiled);
+  static const void* force[8] = PEF_VARIANTS(k_force_tiled);
+  if (which == kTileNlist) return reinterpret_cast<const void*>(&k_nlist_tiled);
+  const int v = variant_of(lpp, bs);
+  if (v < 0) return nullptr;
   switch (which) {
-    case kTileNlist: return reinterpret_cast<const void*>(&k_nlist_tiled);
-    case kTileDensity:
-      return pick(lpp, density_fn<2>(), density_fn<4>(), density_fn<8>(), density_fn<16>());
-    case kTileXsph: return pick(lpp, xsph_fn<2>(), xsph_fn<4>(), xsph_fn<8>(), xsph_fn<16>());
-    case kTileForce: return pick(lpp, force_fn<2>(), force_fn<4>(), force_fn<8>(), force_fn<16>());
+    case kTileDensity: return dens[v];
+    case kTileXsph: return xsph[v];
+    case kTileForce: return force[v];
     default: return nullptr;
   }
 }
@@ -896,7 +944,7 @@ int tiles_configure(TileConfig& c) {
     if (cap < 1) cap = 1;
     c.lds_cap[k] = (uint32_t)cap;
     const uint32_t bs = (k == kTileNlist) ? c.nlist_bs : c.bs;
-    const void* fn = kernel_fn(k, c.lpp);
+    const void* fn = kernel_fn(k, c.lpp, bs);
     if (!fn) return 2;
     const size_t bytes = lds_bytes(c.lds_cap[k], narr);
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 3;
@@ -929,23 +977,29 @@ void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta,
                  with_boundary, always);
 }
 
-#define PEF_LPP_SWITCH(lpp, KERNEL, ...)                \
-  switch (lpp) {                                        \
-    case 2: KERNEL(2, __VA_ARGS__); break;              \
-    case 4: KERNEL(4, __VA_ARGS__); break;              \
-    case 8: KERNEL(8, __VA_ARGS__); break;              \
-    default: KERNEL(16, __VA_ARGS__); break;            \
-  }
+// Launch the instantiation (lanes per particle, largest workgroup) that
+// matches the configuration; launch arguments are passed through.
+#define PEF_LAUNCH_VARIANT(K, c, grid, bytes, st, ...)                                              \
+  do {                                                                                              \
+    const dim3 g_(grid), b_((c).bs);                                                                \
+    switch (variant_of((c).lpp, (c).bs)) {                                                          \
+      case 0: hipLaunchKernelGGL((K<2, 512>), g_, b_, bytes, st, __VA_ARGS__); break;              \
+      case 1: hipLaunchKernelGGL((K<4, 512>), g_, b_, bytes, st, __VA_ARGS__); break;              \
+      case 2: hipLaunchKernelGGL((K<8, 512>), g_, b_, bytes, st, __VA_ARGS__); break;              \
+      case 3: hipLaunchKernelGGL((K<16, 512>), g_, b_, bytes, st, __VA_ARGS__); break;             \
+      case 4: hipLaunchKernelGGL((K<2, 1024>), g_, b_, bytes, st, __VA_ARGS__); break;             \
+      case 5: hipLaunchKernelGGL((K<4, 1024>), g_, b_, bytes, st, __VA_ARGS__); break;             \
+      case 6: hipLaunchKernelGGL((K<8, 1024>), g_, b_, bytes, st, __VA_ARGS__); break;             \
+      default: hipLaunchKernelGGL((K<16, 1024>), g_, b_, bytes, st, __VA_ARGS__); break;           \
+    }                                                                                               \
+  } while (0)
 
 void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
                           const SceneState* ss, const State& cur) {
   TileArgs a = ta;
   a.lds_cap = c.lds_cap[kTileDensity];
   a.queue = ta.queue + 16 * kTileDensity;
-  const size_t bytes = lds_bytes(a.lds_cap, 1);
-#define K_DENS(L, ...) PEF_LAUNCH_LDS(k_density_tiled<L>, c.grid[kTileDensity], c.bs, bytes, st, a, sc, ss, cur)
-  PEF_LPP_SWITCH(c.lpp, K_DENS, 0)
-#undef K_DENS
+  PEF_LAUNCH_VARIANT(k_density_tiled, c, c.grid[kTileDensity], lds_bytes(a.lds_cap, 1), st, a, sc, ss, cur);
 }
 
 void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
@@ -953,10 +1007,7 @@ void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, 
   TileArgs a = ta;
   a.lds_cap = c.lds_cap[kTileXsph];
   a.queue = ta.queue + 16 * kTileXsph;
-  const size_t bytes = lds_bytes(a.lds_cap, 2);
-#define K_XSPH(L, ...) PEF_LAUNCH_LDS(k_xsph_tiled<L>, c.grid[kTileXsph], c.bs, bytes, st, a, sc, cur)
-  PEF_LPP_SWITCH(c.lpp, K_XSPH, 0)
-#undef K_XSPH
+  PEF_LAUNCH_VARIANT(k_xsph_tiled, c, c.grid[kTileXsph], lds_bytes(a.lds_cap, 2), st, a, sc, cur);
 }
 
 void launch_force_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
@@ -965,12 +1016,8 @@ void launch_force_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta,
   TileArgs a = ta;
   a.lds_cap = c.lds_cap[kTileForce];
   a.queue = ta.queue + 16 * kTileForce;
-  const size_t bytes = lds_bytes(a.lds_cap, 2);
-#define K_FORCE(L, ...)                                                                                      \
-  PEF_LAUNCH_LDS(k_force_tiled<L>, c.grid[kTileForce], c.bs, bytes, st, a, ctl, ctlw, sc, ss, cur, integrate, \
-                 with_boundary)
-  PEF_LPP_SWITCH(c.lpp, K_FORCE, 0)
-#undef K_FORCE
+  PEF_LAUNCH_VARIANT(k_force_tiled, c, c.grid[kTileForce], lds_bytes(a.lds_cap, 2), st, a, ctl, ctlw, sc, ss, cur,
+                     integrate, with_boundary);
 }
 
 void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint16_t* LIST,

@@ -290,3 +290,15 @@ def test_errors_and_bounds():
     with pytest.raises(pef.PefError) as e:
         W.step(SMALL.dt)
     assert e.value.code == pef.ERR_GRID
+
+
+def test_list_capacity_overflow_is_reported():
+    # A list capacity below the neighbour count must surface as the sticky
+    # PEF_ERR_NEIGHBOURS, never as a silently truncated sum.
+    W, _ = gpu_world(MED, pef.Options(neighbour_capacity=32))
+    with pytest.raises(pef.PefError) as e:
+        W.step(MED.dt)
+    assert e.value.code == pef.ERR_NEIGHBOURS
+    with pytest.raises(pef.PefError) as e:
+        gpu_world(MED, pef.Options(neighbour_capacity=5000))  # above the 4096 limit
+    assert e.value.code == pef.ERR_ARG
