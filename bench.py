@@ -54,17 +54,26 @@ def step_bytes(n, n_fluid, n_cells, k_mean):
     return 120 * n + (196 + 16 * k_mean) * n_fluid + 8 * n_cells
 
 
-def mean_support_neighbours(x, kind, h, sample=20000, seed=0):
-    """Mean count of particles within 2h (self excluded) of a fixed random
-    sample of fluid particles, from the device positions."""
+def mean_support_neighbours(x, kind, h, ranges=None, sample=20000, seed=0):
+    """Mean count of particles within 2h (self excluded, same scene only) of a
+    fixed random sample of fluid particles, from the device positions.
+    `ranges` lists the (first, count) slot range of each scene of a batch."""
     import numpy as np
     from scipy.spatial import cKDTree
-    fl = np.nonzero(kind == 0)[0]
     rng = np.random.default_rng(seed)
-    pick = rng.choice(fl, size=min(sample, len(fl)), replace=False)
-    tree = cKDTree(x)
-    counts = tree.query_ball_point(x[pick], r=2.0 * h, return_length=True)
-    return float(np.mean(counts) - 1.0)
+    ranges = ranges or [(0, len(x))]
+    per = max(1, sample // len(ranges))
+    total, n = 0.0, 0
+    for a, c in ranges:
+        xs, ks = x[a:a + c], kind[a:a + c]
+        fl = np.nonzero(ks == 0)[0]
+        if len(fl) == 0:
+            continue
+        pick = rng.choice(fl, size=min(per, len(fl)), replace=False)
+        counts = cKDTree(xs).query_ball_point(xs[pick], r=2.0 * h, return_length=True)
+        total += float(np.sum(counts) - len(pick))
+        n += len(pick)
+    return total / max(n, 1)
 
 
 def profiled_traffic(kernel, workload):
@@ -112,7 +121,7 @@ def cpu_baseline(cfg, seconds_hint=10.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the configuration's)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="dam_break_1m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -120,6 +129,8 @@ def main():
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--lpp", type=int, default=0, help="lanes per particle of the pair kernels (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size of the pair kernels (0 = default)")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="time every launch shape for this many steps first and keep the fastest")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this size (counter calibration)")
     args = ap.parse_args()
@@ -132,12 +143,38 @@ def main():
     import pef
     from pef import scenes
 
-    cfg = scenes.CONFIGS[args.config]
+    opts = pef.Options(skin=args.skin, lanes_per_particle=args.lpp, block_size=args.block)
     t_setup = time.perf_counter()
-    parts, params = scenes.build(cfg)
-    W = pef.WcsphWorld(parts, params, pef.Options(skin=args.skin, lanes_per_particle=args.lpp, block_size=args.block))
-    n_fluid = parts.fluid_count()
-    n_bnd = len(parts) - n_fluid
+    tune = None
+    if args.config == scenes.BATCH_256:
+        cfgs = scenes.batch_configs()
+        cfg = cfgs[0]
+        built = [scenes.build(c) for c in cfgs]
+        W = pef.WcsphWorld([p for p, _ in built], [q for _, q in built], opts)
+        kinds = np.concatenate([p.kind for p, _ in built])
+        n_fluid = int(np.count_nonzero(kinds == 0))
+        n_bnd = len(kinds) - n_fluid
+        parts = None
+        h = cfg.h
+    else:
+        cfg = scenes.CONFIGS[args.config]
+        parts, params = scenes.build(cfg)
+        W = pef.WcsphWorld(parts, params, opts)
+        kinds = parts.kind
+        n_fluid = parts.fluid_count()
+        n_bnd = len(parts) - n_fluid
+        h = cfg.h
+    if args.config == scenes.BATCH_256:
+        workload = (f"{args.config}: 256 independent dam breaks in one world, {n_fluid} fluid + {n_bnd} Akinci wall "
+                    f"particles, counts log-uniform in [2048, 65536] and aspects uniform in [0.5, 2] from "
+                    f"xoshiro256** seed 1, dx {cfg.dx}, h {cfg.h}, dt {cfg.dt}, no jitter")
+    else:
+        workload = (f"{cfg.name}: {n_fluid} fluid + {n_bnd} Akinci wall particles, dx {cfg.dx}, h {cfg.h}, "
+                    f"dt {cfg.dt}, tank {cfg.tank}, jitter {cfg.jitter} dx seed {cfg.seed}")
+    if args.steps is None:
+        args.steps = cfg.steps
+    if args.autotune:
+        tune = W.autotune(cfg.dt, args.autotune)
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f} s: {n_fluid} fluid + {n_bnd} boundary particles", file=sys.stderr, flush=True)
 
@@ -171,7 +208,8 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}_tiled",
             "bytes_per_launch": int(kb[dom]), "avg_launch_ms": round(dom_ms, 5)}
-    k_mean = mean_support_neighbours(W.positions(), parts.kind, cfg.h)
+    ranges = [W.scene_range(k) for k in range(W.n_scenes)] if parts is None else None
+    k_mean = mean_support_neighbours(W.positions(), kinds, h, ranges)
     n_cells = tiles["bricks"] * 4
     sb = step_bytes(n_fluid + n_bnd, n_fluid, n_cells, k_mean)
     step_roof = {"bytes_per_step": int(sb), "mean_neighbours_within_2h": round(k_mean, 1),
@@ -190,8 +228,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{cfg.name}: {n_fluid} fluid + {n_bnd} Akinci wall particles, dx {cfg.dx}, "
-                               f"h {cfg.h}, dt {cfg.dt}, tank {cfg.tank}, jitter 0.01 dx seed {cfg.seed}",
+        "config": {"workload": workload,
                    "n_fluid": n_fluid, "n_boundary": n_bnd, "skin": args.skin or DEFAULT_SKIN,
                    "mean_list_entries": round(mean_entries, 1), "list_rebuilds_in_profile": rebuilds1 - rebuilds0,
                    "surrogate_speedup": None},
@@ -206,7 +243,9 @@ def main():
         pef.lib().pefx_calibration_copy.restype = C.c_int32
         pef.lib().pefx_calibration_copy.argtypes = [C.c_uint64]
         assert pef.lib().pefx_calibration_copy(args.calibration_copy_mib << 20) == 0
-    if not args.no_cpu_baseline:
+    if tune is not None:
+        out["autotune"] = tune
+    if not args.no_cpu_baseline and parts is not None:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))
 

@@ -105,7 +105,16 @@ typedef struct PefOptions {
   uint32_t lds_halo_capacity;  /* upper bound on the halo particles a pair kernel stages in LDS
                                   per brick (0 = as many as LDS holds); larger halos read their
                                   partners from global memory, with identical results */
+  uint32_t list_block_size;    /* threads per workgroup of the list build (0 = tuned default) */
 } PefOptions;
+
+/* One launch shape of the pair and list kernels, with its measured step time. */
+typedef struct PefTuneEntry {
+  uint32_t block_size;          /* threads per workgroup, pair kernels */
+  uint32_t lanes_per_particle;  /* lanes sharing one particle's list */
+  uint32_t list_block_size;     /* threads per workgroup, list build */
+  double ms_per_step;           /* mean time of one step on this world's scenes */
+} PefTuneEntry;
 
 /* Global conservation and error diagnostics, reduced in FP64 on the device
  * (diagnostics.rs:19-102, wcsph.rs:350-369). */
@@ -215,6 +224,16 @@ int32_t pef_world_diagnostics(PefWorld* world, uint32_t scene, PefDiagnostics* o
 int32_t pef_world_set_profiling(PefWorld* world, int32_t on);
 int32_t pef_world_kernel_times(PefWorld* world, const char** names, double* ms, uint32_t cap,
                                uint32_t* n);
+/* Autotuner; no reference counterpart. Times every supported launch shape
+ * (workgroup sizes 512 and 1024, 2/4/8/16 lanes per particle, list-build
+ * workgroups 256/512/1024) for `steps` steps of dt on this world's own scenes,
+ * each trial starting from a snapshot of the current state, and keeps the
+ * fastest. The state after the call equals the state before it. `table`
+ * (cap entries, may be NULL) receives every trial and *n their number; *best
+ * the shape kept. Launch shapes change only the FP32 summation order. */
+int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEntry* table, uint32_t cap,
+                           uint32_t* n, PefTuneEntry* best);
+
 /* Mean neighbour-list occupancy (entries per particle within the list radius)
  * and number of list rebuilds since creation. */
 int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds);

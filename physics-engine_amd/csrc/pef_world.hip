@@ -237,6 +237,7 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     if (opt->use_graph < 0) w.use_graph = 0;
     if (opt->block_size) w.tc.bs = opt->block_size;
     if (opt->lanes_per_particle) w.tc.lpp = opt->lanes_per_particle;
+    if (opt->list_block_size) w.tc.nlist_bs = opt->list_block_size;
     w.tc.lds_cap_limit = opt->lds_halo_capacity;
   }
   if (!(w.skin >= 0.0 && w.skin <= 1.0) || w.tc.bs % 64 != 0 || w.tc.bs > 1024 || w.tc.bs < 64 || w.margin > 1024)
@@ -992,6 +993,162 @@ int32_t guard(F&& f) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// autotuner: times the supported launch shapes on the world's own scenes
+// ---------------------------------------------------------------------------
+// Device copy of the per-slot state, for trials that must leave the world as
+// they found it.
+struct Snapshot {
+  pef::State s{};
+  std::vector<void*> bufs;
+  pef::Ctl ctl{};
+  std::vector<pef::SceneState> st;
+  double t = 0.0;
+  ~Snapshot() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
+};
+
+template <class T>
+static hipError_t snap_copy(World& w, Snapshot& sn, T*& dst, const T* src, size_t count, bool to_snap) {
+  if (to_snap) {
+    hipError_t e = hipMalloc((void**)&dst, count * sizeof(T));
+    if (e != hipSuccess) return e;
+    sn.bufs.push_back((void*)dst);
+    return hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, w.stream);
+  }
+  return hipMemcpyAsync(const_cast<T*>(src), dst, count * sizeof(T), hipMemcpyDeviceToDevice, w.stream);
+}
+
+// to_snap: world -> snapshot (allocating); else snapshot -> world.
+static hipError_t snap_state(World& w, Snapshot& sn, bool to_snap) {
+  const size_t N = w.N, N1 = (size_t)w.N + 1;
+  pef::State& c = w.cur;
+  pef::State& d = sn.s;
+  hipError_t e = hipSuccess;
+#define SNAP(f, n) \
+  if (e == hipSuccess) e = snap_copy(w, sn, d.f, c.f, n, to_snap)
+  SNAP(X, N);
+  SNAP(Y, N);
+  SNAP(Z, N);
+  SNAP(A, N1);
+  SNAP(A2, N1);
+  SNAP(B, N1);
+  SNAP(B2, N1);
+  SNAP(ID, N);
+  SNAP(IDF, N);
+  SNAP(META, N);
+  SNAP(CELL, N);
+  SNAP(DISP, N);
+  SNAP(NX, N);
+  SNAP(NY, N);
+  SNAP(NZ, N);
+  SNAP(ACC, N);
+  SNAP(BF, N);
+#undef SNAP
+  if (to_snap) {
+    sn.st.resize(w.S);
+    if (e == hipSuccess) e = hipMemcpyAsync(&sn.ctl, w.ctl_d, sizeof(pef::Ctl), hipMemcpyDeviceToHost, w.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sn.st.data(), w.st_d, w.S * sizeof(pef::SceneState), hipMemcpyDeviceToHost, w.stream);
+    sn.t = w.t;
+  } else {
+    if (e == hipSuccess) e = hipMemcpyAsync(w.ctl_d, &sn.ctl, sizeof(pef::Ctl), hipMemcpyHostToDevice, w.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(w.st_d, sn.st.data(), w.S * sizeof(pef::SceneState), hipMemcpyHostToDevice, w.stream);
+    w.t = sn.t;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  return e;
+}
+
+// Restore the snapshot and rebuild the neighbour lists for the restored
+// positions. The restored densities and pressures belong to those positions
+// (the snapshot is taken with them valid), so they are kept as they were.
+static int32_t restore_and_rebuild(World& w, Snapshot& sn) {
+  HIPTRY(snap_state(w, sn, false));
+  w.probe_valid = false;
+  int32_t rc = set_force_rebuild(w);
+  if (rc) return rc;
+  enqueue_rebuild(w, nullptr);
+  HIPTRY(hipGetLastError());
+  w.density_valid = true;
+  return read_flags(w);
+}
+
+int32_t autotune(World& w, double dt, uint32_t steps, PefTuneEntry* table, uint32_t cap, uint32_t* n,
+                 PefTuneEntry* best) {
+  if (!(std::isfinite(dt) && dt > 0.0) || steps == 0 || steps > 1000) return PEF_ERR_ARG;
+  int32_t rc = ensure_density(w);
+  if (rc) return rc;
+  HIPTRY(hipMemcpyAsync((char*)w.ctl_d + offsetof(pef::Ctl, dt), &dt, sizeof(double), hipMemcpyHostToDevice,
+                        w.stream));
+  Snapshot sn;
+  HIPTRY(snap_state(w, sn, true));
+  const pef::TileConfig keep = w.tc;
+  static const uint32_t kBs[] = {512, 1024};
+  static const uint32_t kLpp[] = {2, 4, 8, 16};
+  static const uint32_t kListBs[] = {256, 512, 1024};
+  std::vector<PefTuneEntry> trials;
+  hipEvent_t e0, e1;
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  int32_t out_rc = PEF_OK;
+  for (uint32_t bs : kBs) {
+    for (uint32_t lpp : kLpp) {
+      for (uint32_t lbs : kListBs) {
+        pef::TileConfig c = keep;
+        c.bs = bs;
+        c.lpp = lpp;
+        c.nlist_bs = lbs;
+        if (pef::tiles_configure(c) != 0) continue;
+        w.tc = c;
+        rc = restore_and_rebuild(w, sn);
+        if (rc) { out_rc = rc; break; }
+        enqueue_body(w, nullptr);  // warm-up step
+        HIPTRY(hipEventRecord(e0, w.stream));
+        for (uint32_t k = 0; k < steps; ++k) enqueue_body(w, nullptr);
+        HIPTRY(hipEventRecord(e1, w.stream));
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+        rc = read_flags(w);
+        if (rc) { out_rc = rc; break; }
+        trials.push_back(PefTuneEntry{bs, lpp, lbs, (double)ms / steps});
+      }
+      if (out_rc) break;
+    }
+    if (out_rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // keep the fastest shape (the configured one if every trial failed)
+  pef::TileConfig chosen = keep;
+  PefTuneEntry pick{keep.bs, keep.lpp, keep.nlist_bs, 0.0};
+  for (const PefTuneEntry& t : trials)
+    if (pick.ms_per_step == 0.0 || t.ms_per_step < pick.ms_per_step) pick = t;
+  chosen.bs = pick.block_size;
+  chosen.lpp = pick.lanes_per_particle;
+  chosen.nlist_bs = pick.list_block_size;
+  if (pef::tiles_configure(chosen) != 0) chosen = keep;
+  w.tc = chosen;
+  if (w.graph) {
+    (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+  }
+  rc = restore_and_rebuild(w, sn);
+  if (out_rc) return out_rc;
+  if (rc) return rc;
+  if (n) *n = (uint32_t)trials.size();
+  if (best) *best = pick;
+  if (table) {
+    if (cap < trials.size()) return PEF_ERR_BOUNDS;
+    std::memcpy(table, trials.data(), trials.size() * sizeof(PefTuneEntry));
+  }
+  return PEF_OK;
+}
+
 extern "C" {
 
 const char* pef_version_string(void) { return "pef 0.1.0 (gfx950 WCSPH)"; }
@@ -1175,6 +1332,12 @@ int32_t pef_world_kernel_times(PefWorld* world, const char** names, double* ms, 
     ms[k] = w.prof_ms[k];
   }
   return PEF_OK;
+}
+
+int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEntry* table, uint32_t cap,
+                           uint32_t* n, PefTuneEntry* best) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return autotune(world->w, dt, steps, table, cap, n, best); });
 }
 
 int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds) {

@@ -80,7 +80,21 @@ class PefOptions(C.Structure):
         ("block_size", C.c_uint32),
         ("lanes_per_particle", C.c_uint32),
         ("lds_halo_capacity", C.c_uint32),
+        ("list_block_size", C.c_uint32),
     ]
+
+
+class PefTuneEntry(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_uint32),
+        ("lanes_per_particle", C.c_uint32),
+        ("list_block_size", C.c_uint32),
+        ("ms_per_step", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {"block_size": self.block_size, "lanes_per_particle": self.lanes_per_particle,
+                "list_block_size": self.list_block_size, "ms_per_step": self.ms_per_step}
 
 
 class PefDiagnostics(C.Structure):
@@ -304,6 +318,7 @@ class Options:
     block_size: int = 0
     lanes_per_particle: int = 0  # 0 = library default
     lds_halo_capacity: int = 0  # 0 = as many as LDS holds
+    list_block_size: int = 0  # 0 = library default
 
     def to_c(self) -> PefOptions:
         o = PefOptions()
@@ -315,6 +330,7 @@ class Options:
         o.block_size = self.block_size
         o.lanes_per_particle = self.lanes_per_particle
         o.lds_halo_capacity = self.lds_halo_capacity
+        o.list_block_size = self.list_block_size
         return o
 
 
@@ -431,6 +447,12 @@ class WcsphWorld:
     def sound_speed(self) -> float:
         return self.get_params(0).sound_speed
 
+    @property
+    def n_scenes(self) -> int:
+        k = C.c_uint32()
+        _check(lib().pef_world_scene_count(self._h, C.byref(k)), "pef_world_scene_count")
+        return k.value
+
     def scene_range(self, scene: int):
         a, b = C.c_uint64(), C.c_uint64()
         _check(lib().pef_world_scene_range(self._h, scene, C.byref(a), C.byref(b)), "scene_range")
@@ -494,6 +516,20 @@ class WcsphWorld:
         m, r = C.c_double(), C.c_uint64()
         _check(lib().pef_world_list_stats(self._h, C.byref(m), C.byref(r)), "list_stats")
         return m.value, r.value
+
+    def autotune(self, dt: float, steps: int = 5) -> dict:
+        """Time every supported launch shape on this world's scenes and keep the
+        fastest (pef_world_autotune); the simulation state is unchanged."""
+        cap = 64
+        table = (PefTuneEntry * cap)()
+        best = PefTuneEntry()
+        n = C.c_uint32()
+        f = lib().pef_world_autotune
+        f.restype = C.c_int32
+        f.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(PefTuneEntry), C.c_uint32,
+                      C.POINTER(C.c_uint32), C.POINTER(PefTuneEntry)]
+        _check(f(self._h, dt, steps, table, cap, C.byref(n), C.byref(best)), "pef_world_autotune")
+        return {"best": best.as_dict(), "trials": [table[i].as_dict() for i in range(n.value)]}
 
     def tile_info(self) -> dict:
         """Launch configuration of the brick-tiled kernels and brick counts."""

@@ -22,10 +22,54 @@ Every parameter the configurations leave open is fixed here once:
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, replace
-from typing import Tuple
+from typing import List, Tuple
 
 import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+class Xoshiro256StarStar:
+    """xoshiro256** seeded through SplitMix64, with uniform(lo, hi) from the
+    high 53 bits, as the reference's scene generators draw
+    (crates/pe-math/src/rng.rs:35-70)."""
+
+    def __init__(self, seed: int):
+        sm = seed & _M64
+        s = []
+        for _ in range(4):
+            sm = (sm + 0x9E3779B97F4A7C15) & _M64
+            z = sm
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+            s.append(z ^ (z >> 31))
+        if not any(s):
+            s[0] = 0x9E3779B97F4A7C15
+        self.s = s
+
+    @staticmethod
+    def _rotl(x: int, k: int) -> int:
+        return ((x << k) | (x >> (64 - k))) & _M64
+
+    def next_u64(self) -> int:
+        s = self.s
+        result = (self._rotl((s[1] * 5) & _M64, 7) * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return result
+
+    def next_f64(self) -> float:
+        return float(self.next_u64() >> 11) * (1.0 / float(1 << 53))
+
+    def uniform(self, lo: float, hi: float) -> float:
+        return lo + (hi - lo) * self.next_f64()
 
 
 @dataclass(frozen=True)
@@ -69,6 +113,42 @@ GPU_1M = DamBreak("dam_break_1m", 128, 64, 128, 0.005, (1.6, 0.335, 0.8), 2.5e-5
 GPU_8M = DamBreak("dam_break_8m", 256, 128, 256, 0.005, (3.2, 0.655, 1.6), 2.5e-5, 2000)
 
 CONFIGS = {c.name: c for c in (CPU_8K, CHECK_64K, GPU_1M, GPU_8M)}
+
+
+def _round_half_away(x: float) -> int:
+    return int(math.floor(x + 0.5))
+
+
+def batch_configs(n_scenes: int = 256, seed: int = 1, dx: float = 0.01, dt: float = 5e-5,
+                  steps: int = 200) -> List[DamBreak]:
+    """Independent dam-break scenes of uneven size for one batched world.
+
+    Per scene, in order, from xoshiro256** seeded with `seed`: a target count
+    N = exp(uniform(ln 2048, ln 65536)) (log-uniform) and an aspect
+    a = uniform(0.5, 2). The block is nx = round(cbrt(N a)) by ny = nz =
+    round(sqrt(N / nx)) particles at spacing dx, h = 2 dx; the realised count
+    is nx ny nz. The tank is 2.5 Lx long and 1.25 Lz high (L = n dx, rounded
+    up to whole dx), with the block 2 dx from the walls and spanning the tank
+    width. No jitter. dt 5e-5 s keeps the 1M dam break's acoustic CFL number
+    c dt / h = 0.05 at dx = 0.01 m.
+    """
+    rng = Xoshiro256StarStar(seed)
+    out = []
+    gap = 2.0
+    for k in range(n_scenes):
+        n_target = math.exp(rng.uniform(math.log(2048.0), math.log(65536.0)))
+        aspect = rng.uniform(0.5, 2.0)
+        nx = max(1, _round_half_away((n_target * aspect) ** (1.0 / 3.0)))
+        ny = max(1, _round_half_away(math.sqrt(n_target / nx)))
+        nz = ny
+        tx = max(math.ceil(2.5 * nx), nx - 1 + 2 * int(gap) + 1)
+        tz = max(math.ceil(1.25 * nz), nz - 1 + 2 * int(gap))
+        tank = (round(tx * dx, 10), round((ny - 1 + 2 * gap) * dx, 10), round(tz * dx, 10))
+        out.append(DamBreak(f"batch_{k}", nx, ny, nz, dx, tank, dt, steps, jitter=0.0))
+    return out
+
+
+BATCH_256 = "batch_256"
 
 
 def small(nx=12, ny=8, nz=10, dx=0.01, steps=20, jitter=0.01, dim=3) -> DamBreak:

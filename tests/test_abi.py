@@ -104,3 +104,19 @@ def test_scene_configs_particle_counts():
         dims = tuple(int(round(L / cfg.dx)) + 1 for L in cfg.tank)
         assert shell(dims) == nb
     assert scenes.GPU_1M.n_fluid == 1048576 and scenes.CHECK_64K.n_fluid == 65536
+
+
+def test_batch_scene_generator_draws_like_the_reference_rng():
+    from pef import scenes
+    r1, r2 = scenes.Xoshiro256StarStar(1), O.Rng(1)
+    for _ in range(2000):
+        assert r1.uniform(-3.0, 7.0) == r2.uniform(-3.0, 7.0)
+    cfgs = scenes.batch_configs()
+    counts = [c.n_fluid for c in cfgs]
+    assert len(cfgs) == 256 and sum(counts) == 4327299
+    assert (cfgs[0].nx, cfgs[0].ny, cfgs[0].nz) == (31, 27, 27)
+    assert min(counts) >= 1800 and max(counts) <= 70000
+    for c in cfgs:  # block inside the tank with the 2 dx wall gap
+        assert (c.nx - 1 + 4) * c.dx <= c.tank[0] + 1e-12
+        assert abs((c.ny - 1 + 4) * c.dx - c.tank[1]) < 1e-9
+        assert (c.nz - 1 + 4) * c.dx <= c.tank[2] + 1e-12

@@ -302,3 +302,30 @@ def test_list_capacity_overflow_is_reported():
     with pytest.raises(pef.PefError) as e:
         gpu_world(MED, pef.Options(neighbour_capacity=5000))  # above the 4096 limit
     assert e.value.code == pef.ERR_ARG
+
+
+def test_autotune_restores_state_and_its_choice_reproduces_a_fresh_world():
+    cfgs = [scenes.small(10, 6, 8, dx=0.01), scenes.small(14, 8, 9, dx=0.01, jitter=0.02)]
+    built = [scenes.build(c) for c in cfgs]
+    dt = cfgs[0].dt
+    # the state after tuning is the state before it
+    C = pef.WcsphWorld([p for p, _ in built], [q for _, q in built])
+    C.step(dt, 3)
+    x0, v0, r0 = C.positions(), C.velocities(), C.densities()
+    res = C.autotune(dt, 2)
+    assert len(res["trials"]) == 24 and res["best"]["ms_per_step"] > 0
+    for a, b in ((C.positions(), x0), (C.velocities(), v0), (C.densities(), r0)):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert C.tile_info()["lanes_per_particle"] == res["best"]["lanes_per_particle"]
+    # a tuned world steps exactly like a fresh world created with the chosen shape
+    B = pef.WcsphWorld([p for p, _ in built], [q for _, q in built])
+    best = B.autotune(dt, 2)["best"]
+    # the initial densities were summed with the default shape: re-stage so that
+    # both worlds start from densities summed with the chosen one
+    B.set_state(x=B.positions(), v=B.velocities())
+    B.step(dt, 8)
+    opts = pef.Options(block_size=best["block_size"], lanes_per_particle=best["lanes_per_particle"],
+                       list_block_size=best["list_block_size"])
+    F = pef.WcsphWorld([p for p, _ in built], [q for _, q in built], opts)
+    F.step(dt, 8)
+    assert np.array_equal(B.positions().view(np.uint64), F.positions().view(np.uint64))
