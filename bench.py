@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--lpp", type=int, default=0, help="lanes per particle of the pair kernels (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size of the pair kernels (0 = default)")
+    ap.add_argument("--lds-cap", type=int, default=0, help="upper bound on the LDS halo capacity (0 = default)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="time every launch shape for this many steps first and keep the fastest")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -143,7 +144,8 @@ def main():
     import pef
     from pef import scenes
 
-    opts = pef.Options(skin=args.skin, lanes_per_particle=args.lpp, block_size=args.block)
+    opts = pef.Options(skin=args.skin, lanes_per_particle=args.lpp, block_size=args.block,
+                       lds_halo_capacity=args.lds_cap)
     t_setup = time.perf_counter()
     tune = None
     if args.config == scenes.BATCH_256:

@@ -248,86 +248,86 @@ __device__ inline Own own_of(const HaloTab& T, uint32_t p) {
 }
 
 // ---------------------------------------------------------------------------
-// list storage: u16 entries, contiguous per slot, read as 8-byte words of four
-// entries. Each block of 16 entries is stored transposed: logical entry
-// 16 b + 4 k + w sits at 16 b + 4 w + k, so word 4 b + w holds entries
-// 16 b + w + {0, 4, 8, 12}. Lanes that walk consecutive words then read
-// consecutive list entries at each step. Lists are in ascending halo order,
-// so those are nearby LDS records in distinct bank groups, and neighbouring
-// particles, whose lists overlap, read the same records. Lists are padded to
-// whole blocks with kListPad.
+// list storage: u16 entries, contiguous per slot, read as 16-byte chunks of
+// eight entries. Each block of 32 entries is stored transposed: logical entry
+// 32 b + 4 k + c sits at 32 b + 8 c + k (k < 8, c < 4), so chunk 4 b + c holds
+// entries 32 b + c + {0, 4, ..., 28}. The four lanes of a particle read the
+// four chunks of a block (one 64-byte run) and, at each step, consecutive list
+// entries: lists are in ascending halo order, so those are nearby LDS records
+// in distinct bank groups, and neighbouring particles, whose lists overlap,
+// read the same records. Lists are padded to whole blocks with kListPad.
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t list_pos(uint32_t k) { return (k & ~15u) | ((k & 3u) << 2) | ((k >> 2) & 3u); }
-__device__ inline uint32_t list_words_used(uint32_t cnt) { return ((cnt + 15u) >> 4) << 2; }
-__device__ inline const uint2* list_words(const uint16_t* LIST, uint32_t s, uint32_t cap) {
-  return reinterpret_cast<const uint2*>(LIST + (uint64_t)s * cap);
+constexpr uint32_t kListBlock = 32;
+__device__ inline uint32_t list_pos(uint32_t k) { return (k & ~31u) | ((k & 3u) << 3) | ((k >> 2) & 7u); }
+__device__ inline uint32_t list_chunks_used(uint32_t cnt) { return ((cnt + 31u) >> 5) << 2; }
+__device__ inline const uint4* list_chunks(const uint16_t* LIST, uint32_t s, uint32_t cap) {
+  return reinterpret_cast<const uint4*>(LIST + (uint64_t)s * cap);
 }
-typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 // Lists are streamed once per pass: non-temporal loads keep them from evicting
 // the halo data that neighbouring bricks stage through L2.
-__device__ inline uint2 load_word(const uint2* lw, uint32_t g) {
-  const u32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(lw + g));
-  return make_uint2(v.x, v.y);
+__device__ inline uint4 load_chunk(const uint4* lc, uint32_t g) {
+  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(lc + g));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void chunk_entries(const uint4& c, uint32_t e[8]) {
+  e[0] = c.x & 0xFFFFu;
+  e[1] = c.x >> 16;
+  e[2] = c.y & 0xFFFFu;
+  e[3] = c.y >> 16;
+  e[4] = c.z & 0xFFFFu;
+  e[5] = c.z >> 16;
+  e[6] = c.w & 0xFFFFu;
+  e[7] = c.w >> 16;
 }
 
-// Walk list words sub, sub + LPP, ... < ng; fn(word) evaluates four entries.
-// The loads of the next two words are in flight while a word is evaluated.
-// They are unconditional (indices clamped to the last word) so the compiler
-// can wait for the oldest load only, not for every load outstanding.
-// walk_words2 evaluates two words (eight entries) per step: all eight partner
-// loads are issued before the first pair is evaluated, which hides LDS
-// latency; a missing second word at the end is replaced by padding (the
-// sentinel). walk_words1 takes one word per step, for pair bodies whose
-// registers would not hold eight partners.
-template <int LPP, class Fn>
-__device__ inline void walk_words2(const uint2* lw, uint32_t ng, uint32_t sub, uint2 a, uint2 b, Fn fn) {
-  if (sub >= ng) return;
-  const uint32_t last = ng - 1u;
-  const uint2 pad = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-  for (uint32_t g = sub; g < ng; g += 2u * LPP) {
-    const uint2 na = load_word(lw, min(g + 2u * LPP, last));
-    const uint2 nb = load_word(lw, min(g + 3u * LPP, last));
-    fn(a, (g + LPP < ng) ? b : pad);
-    a = na;
-    b = nb;
-  }
-}
-template <int LPP, class Fn>
-__device__ inline void walk_words1(const uint2* lw, uint32_t ng, uint32_t sub, uint2 cur, uint2 nxt, Fn fn) {
-  if (sub >= ng) return;
-  const uint32_t last = ng - 1u;
-  for (uint32_t g = sub; g < ng; g += LPP) {
-    const uint2 nn = load_word(lw, min(g + 2u * LPP, last));
-    fn(cur);
-    cur = nxt;
-    nxt = nn;
+// Walk list chunks sub, sub + LPP, ... < nc; fn(chunk) evaluates eight
+// entries. The loads of the next DEPTH chunks (1 or 2) are in flight while a
+// chunk is evaluated; they are unconditional (indices clamped to the last
+// chunk) so the compiler waits for the oldest load only. Pair bodies that need
+// many registers use DEPTH 1.
+template <int LPP, int DEPTH, class Fn>
+__device__ inline void walk_chunks(const uint4* lc, uint32_t nc, uint32_t sub, uint4 cur, uint4 nxt, Fn fn) {
+  if (sub >= nc) return;
+  const uint32_t last = nc - 1u;
+  for (uint32_t g = sub; g < nc; g += LPP) {
+    if (DEPTH == 2) {
+      const uint4 nn = load_chunk(lc, min(g + 2u * LPP, last));
+      fn(cur);
+      cur = nxt;
+      nxt = nn;
+    } else {
+      const uint4 nn = load_chunk(lc, min(g + LPP, last));
+      fn(cur);
+      cur = nn;
+    }
   }
 }
 
 // One round of a pair kernel: the brick particle of this lane group, its list
-// count and its first two list words. The first round's are loaded before the
+// count and its first two list chunks. The first round's are loaded before the
 // halo is staged, so their latency overlaps the staging.
 struct Round {
   Own o;
   bool act;
   uint32_t cnt;
-  uint2 w0, w1;
-  const uint2* lw;
+  uint4 c0, c1;
+  const uint4* lc;
 };
-template <int LPP>
+template <int LPP, int DEPTH>
 __device__ inline Round round_load(const HaloTab& T, const TileArgs& ta, uint32_t p, uint32_t n_own, uint32_t sub) {
   Round r;
   r.act = p < n_own;
   r.o = Own{0, 0, 0};
   r.cnt = 0;
-  r.w0 = r.w1 = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-  r.lw = nullptr;
+  r.c0 = r.c1 = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+  r.lc = nullptr;
   if (r.act) {
     r.o = own_of(T, p);
-    r.lw = list_words(ta.LIST, r.o.s, ta.cap);
-    const uint32_t last = (ta.cap >> 2) - 1u;  // words beyond the count are read but never used
-    r.w0 = load_word(r.lw, min(sub, last));
-    r.w1 = load_word(r.lw, min(sub + LPP, last));
+    r.lc = list_chunks(ta.LIST, r.o.s, ta.cap);
+    const uint32_t last = (ta.cap >> 3) - 1u;  // chunks beyond the count are read but never used
+    r.c0 = load_chunk(r.lc, min(sub, last));
+    if (DEPTH == 2) r.c1 = load_chunk(r.lc, min(sub + LPP, last));
     r.cnt = ta.CNT[r.o.s];
   }
   return r;
@@ -460,8 +460,8 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
     }
   }
   const uint32_t stored = cnt < cap ? cnt : cap;
-  const uint32_t padn = (16u - (stored & 15u)) & 15u;  // pad the last block
-  if (sub < padn) out[list_pos(stored + sub)] = (uint16_t)kListPad;
+  const uint32_t padn = (kListBlock - (stored & (kListBlock - 1u))) & (kListBlock - 1u);  // pad the last block
+  for (uint32_t j = sub; j < padn; j += kNlistLanes) out[list_pos(stored + j)] = (uint16_t)kListPad;
   return cnt;
 }
 
@@ -521,9 +521,9 @@ template <int LPP, bool FAST>
 __device__ inline float density_walk(const Round& r, uint32_t sub, const float4& ai, float r2s, float qu,
                                      const Src<FAST, 1>& src) {
   float sum = 0.0f, comp = 0.0f;
-  walk_words2<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd, const uint2 wd2) {
-    const uint32_t e[8] = {wd.x & 0xFFFFu,  wd.x >> 16,  wd.y & 0xFFFFu,  wd.y >> 16,
-                           wd2.x & 0xFFFFu, wd2.x >> 16, wd2.y & 0xFFFFu, wd2.y >> 16};
+  walk_chunks<LPP, 2>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
     float4 a[8], unused;
 #pragma unroll
     for (int k = 0; k < 8; ++k) src.get(e[k], a[k], unused);
@@ -533,8 +533,9 @@ __device__ inline float density_walk(const Round& r, uint32_t sub, const float4&
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
-      const float w = fabsf(aj.w) * cubic_w(r * qu, 1.0f);  // weight m (fluid) or rho0 psi (boundary)
-      const float term = (r2 < r2s) ? w : 0.0f;
+      // weight m (fluid) or rho0 psi (boundary); cubic_w is exactly 0 for q >= 2,
+      // so list entries in the skin and the far sentinel add nothing
+      const float term = fabsf(aj.w) * cubic_w(r * qu, 1.0f);
       const float y = term - comp;  // compensated (Kahan) sum
       const float tt = sum + y;
       comp = (tt - sum) - y;
@@ -559,7 +560,7 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
     const bool fast = H <= CAP;
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
-    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
+    Round rd = round_load<LPP, 2>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<1>(Tb, H, R, CAP, cur.A, nullptr);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
     const Src<true, 1> fsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
     const Src<false, 1> gsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      if (base != 0) rd = round_load<LPP, 2>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
       const bool act = rd.act;
       const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), unused;
@@ -603,9 +604,9 @@ template <int LPP, bool FAST>
 __device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi, float r2s,
                                    float qu, float alpha, float two_m, const Src<FAST, 2>& src) {
   float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
-  walk_words2<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd, const uint2 wd2) {
-    const uint32_t e[8] = {wd.x & 0xFFFFu,  wd.x >> 16,  wd.y & 0xFFFFu,  wd.y >> 16,
-                           wd2.x & 0xFFFFu, wd2.x >> 16, wd2.y & 0xFFFFu, wd2.y >> 16};
+  walk_chunks<LPP, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
     float4 a[8], b[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) src.get(e[k], a[k], b[k]);
@@ -616,9 +617,10 @@ __device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& a
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
       const float wij = cubic_w(r * qu, alpha);
-      // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2
+      // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2; fluid partners only
+      // (W is exactly 0 beyond 2h, so the skin and the sentinel add nothing)
       float sfac = two_m * wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
-      sfac = (r2 < r2s && aj.w > 0.0f) ? sfac : 0.0f;  // fluid partners only
+      sfac = (aj.w > 0.0f) ? sfac : 0.0f;
       dvx += (bj.x - bi.x) * sfac;
       dvy += (bj.y - bi.y) * sfac;
       dvz += (bj.z - bi.z) * sfac;
@@ -639,7 +641,7 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
     const bool fast = H <= CAP;
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
-    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
+    Round rd = round_load<LPP, 1>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -648,7 +650,7 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      if (base != 0) rd = round_load<LPP, 1>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
       const bool act = rd.act;
       const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
@@ -696,11 +698,14 @@ __device__ inline float3 force_walk(const Round& r, uint32_t sub, const float4& 
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float Pi = bi.w, rhoi = ai.w;
   const float react = K.m * ai.w;  // boundary i: m (-rho0 psi_i)
-  walk_words1<LPP>(r.lw, list_words_used(r.cnt), sub, r.w0, r.w1, [&](const uint2 wd) {
-    const uint32_t e[4] = {wd.x & 0xFFFFu, wd.x >> 16, wd.y & 0xFFFFu, wd.y >> 16};
+  walk_chunks<LPP, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
+#pragma unroll
+    for (int half = 0; half < 8; half += 4) {  // four partners in flight at a time
     float4 a[4], b[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) src.get(e[k], a[k], b[k]);
+    for (int k = 0; k < 4; ++k) src.get(e[half + k], a[k], b[k]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float4 aj = a[k], bj = b[k];
@@ -708,7 +713,9 @@ __device__ inline float3 force_walk(const Round& r, uint32_t sub, const float4& 
       // gf = -dW/dr / r(units), and mu = (h/unit) v.r / (r^2 + eps h^2/unit^2)
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
-      const float rinv = __builtin_amdgcn_rsqf(r2);
+      // dW/dr is exactly 0 for q >= 2 and at r = 0; clamping r^2 keeps 1/r finite
+      // there, so no cutoff select is needed
+      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
       const float r = r2 * rinv;
       const float gf = -cubic_dwdr(r * K.qu, K.aoh) * rinv;
       float sfac;
@@ -728,10 +735,10 @@ __device__ inline float3 force_walk(const Round& r, uint32_t sub, const float4& 
         const float sf_bnd = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
         sfac = (aj.w > 0.0f) ? sf_fluid : sf_bnd;
       }
-      sfac = (r2 < K.r2s && r2 > 0.0f) ? sfac : 0.0f;
       ax += rx * sfac;
       ay += ry * sfac;
       az += rz * sfac;
+    }
     }
   });
   return make_float3(ax, ay, az);
@@ -761,7 +768,7 @@ __global__ void __launch_bounds__(MAXT) k_force_tiled(TileArgs ta, const Ctl* __
     const bool fast = H <= CAP;
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
-    Round rd = round_load<LPP>(Tb, ta, threadIdx.x / LPP, n_own, sub);
+    Round rd = round_load<LPP, 1>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B2);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -778,7 +785,7 @@ __global__ void __launch_bounds__(MAXT) k_force_tiled(TileArgs ta, const Ctl* __
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      if (base != 0) rd = round_load<LPP>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      if (base != 0) rd = round_load<LPP, 1>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
       const bool act = rd.act;
       const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;

@@ -260,8 +260,8 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
     }
     w.cap = (uint32_t)std::min(4096.0, std::ceil(need / 32.0) * 32.0);
   }
-  w.cap = (w.cap + 15u) & ~15u;  // lists are stored in blocks of 16 entries
-  if (w.cap < 16 || w.cap > 4096) return fail(PEF_ERR_ARG);
+  w.cap = (w.cap + 31u) & ~31u;  // lists are stored in blocks of 32 entries
+  if (w.cap < 32 || w.cap > 4096) return fail(PEF_ERR_ARG);
   w.S = n_scenes;
   w.N = (uint32_t)Ntot;
   w.params.assign(params, params + n_scenes);
