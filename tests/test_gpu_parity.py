@@ -329,3 +329,113 @@ def test_autotune_restores_state_and_its_choice_reproduces_a_fresh_world():
     F = pef.WcsphWorld([p for p, _ in built], [q for _, q in built], opts)
     F.step(dt, 8)
     assert np.array_equal(B.positions().view(np.uint64), F.positions().view(np.uint64))
+
+
+def _box_surface(lo, hi, dx):
+    """Samples of an axis-aligned box surface at spacing dx with outward unit
+    normals (corners and edges take the normalised sum of their faces): the
+    one-sided boundary of a rigid obstacle."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    n = np.rint((hi - lo) / dx).astype(int) + 1
+    pts, nrm = [], []
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                idx = (i, j, k)
+                nv = np.zeros(3)
+                for a in range(3):
+                    if idx[a] == 0:
+                        nv[a] -= 1.0
+                    elif idx[a] == n[a] - 1:
+                        nv[a] += 1.0
+                if not nv.any():
+                    continue
+                pts.append(lo + dx * np.array(idx, float))
+                nrm.append(nv / np.linalg.norm(nv))
+    return np.array(pts), np.array(nrm)
+
+
+def test_moving_one_sided_obstacle_matches_oracle():
+    # The coupling caller pattern (coupling.rs:289-305, 853-865): a rigid
+    # obstacle sampled as one-sided Akinci boundary (outward normals, so the
+    # boundary_sees_fluid filter of particles.rs:126-134 is active) is moved
+    # every step through set_state, and its boundary_force is read back.
+    cfg = scenes.small(16, 8, 10, dx=0.01, steps=30)
+    dx, h = cfg.dx, cfg.h
+    m = O.mass_calibrated_to_rest(cfg.rest_density, dx, h, 3)
+    o = cfg.wall_gap * dx
+    P = O.Particles(3, cfg.rest_density, m, h)
+    P.fill_box((o, o, o), cfg.nx, cfg.ny, cfg.nz, dx)
+    P.jitter(0, 0.01 * dx)
+    P.fill_box_boundary((0.0, 0.0, 0.0), cfg.tank, dx, 1)
+    a = P.arrays()
+    obs_x, obs_n = _box_surface((0.19, 0.03, 0.03), (0.23, 0.07, 0.07), dx)
+    x = np.concatenate([a["x"], obs_x])
+    kind = np.concatenate([a["kind"], np.ones(len(obs_x), np.uint8)])
+    n_out = np.concatenate([np.zeros_like(a["x"]), obs_n])
+    psi0 = np.concatenate([a["psi"], np.full(len(obs_x), dx ** 3)])
+    # psi over all boundary samples (tank + obstacle), by the product and by the oracle
+    G = pef.Particles(3, cfg.rest_density, m, h)
+    G._append(x, np.zeros_like(x), kind, psi0, n_out)
+    pef.recomputed_akinci_psi(G)
+    Q = O.Particles(3, cfg.rest_density, m, h)
+    for i in range(len(x)):
+        if kind[i] == 0:
+            Q.push_fluid(x[i])
+        else:
+            Q.push_boundary(x[i], psi0[i], n_out[i])
+    Q.recompute_psi()
+    psi = Q.arrays()["psi"]
+    assert np.array_equal(G.psi.view(np.uint64), psi.view(np.uint64))
+    params = pef.WcsphParams.water(cfg.rest_density, cfg.sound_speed, cfg.gravity)
+    params.visc_alpha = cfg.visc_alpha
+    params.adapt_sound_speed = False
+    W = pef.WcsphWorld(G, params)
+    prm = dict(dim=3, rest_density=cfg.rest_density, mass=m, h=h, sound_speed=cfg.sound_speed,
+               visc_alpha=cfg.visc_alpha, gravity=cfg.gravity, adapt_sound_speed=False)
+    Wo = O.World(prm, x, kind, psi=psi, n_out=n_out, threads=4)
+    obs = np.arange(len(a["x"]), len(x))
+    vel = np.array([-0.2, 0.0, 0.05])
+    fl = kind == 0
+    for k in range(cfg.steps):
+        xg, vg = W.positions(), W.velocities()
+        so = Wo.state()
+        xo, vo = so["x"].reshape(-1, 3).copy(), so["v"].reshape(-1, 3).copy()
+        step_x = x[obs] + (k + 1) * cfg.dt * vel  # prescribed rigid translation
+        xg[obs], vg[obs] = step_x, vel
+        xo[obs], vo[obs] = step_x, vel
+        W.set_state(x=xg, v=vg)
+        Wo.set_state(x=xo, v=vo)
+        W.step(cfg.dt)
+        Wo.step(cfg.dt)
+        if k % 5 == 4:
+            e = rel_rms(W.densities()[fl], Wo.state()["rho"][fl])
+            assert e <= 1e-5, (k, e)
+    bf, bfo = W.boundary_force(), Wo.state()["boundary_force"].reshape(-1, 3)
+    assert np.abs(bfo[obs]).sum() > 0
+    assert vec_rel_rms(bf[obs], bfo[obs]) <= 1e-4
+    # the one-sided filter removes partners behind the obstacle faces
+    xs = W.positions()
+    near = [i for i in np.nonzero(fl)[0] if np.min(np.linalg.norm(xs[obs] - xs[i], axis=1)) < 2 * h]
+    assert near
+    Wo.set_state(x=xs)
+    Wo.rebuild_grid()
+    filtered = 0
+    for i in near[:40]:
+        got = set(int(j) for j in W.neighbour_list(int(i)))
+        cand = Wo.neighbours(int(i))
+        r = np.linalg.norm(xs[cand] - xs[i], axis=1)
+        want = set()
+        for j, rr in zip(cand, r):
+            j = int(j)
+            if j == i or rr >= 2 * h or abs(rr - 2 * h) <= 1e-6 * h:
+                continue
+            if kind[j] == 1 and n_out[j].any():
+                nn = n_out[j]
+                if np.dot(xs[i] - xs[j], nn) <= -0.15 * h * np.linalg.norm(nn):
+                    filtered += 1
+                    continue
+            want.add(j)
+        edge = set(int(j) for j, rr in zip(cand, r) if abs(rr - 2 * h) <= 1e-6 * h)
+        assert want - edge == got - edge
+    assert filtered > 0  # the one-sided filter was exercised
