@@ -238,6 +238,22 @@ int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEn
  * and number of list rebuilds since creation. */
 int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds);
 
+/* ---- dam-break quantities of interest (validation.rs:63-117) --------------- */
+
+/* column_heights (validation.rs:66-92) of scene k: n_bins = max(1, ceil((x_max -
+ * x_min) / bin_w)) bins of width bin_w over [x_min, x_max); each fluid particle
+ * adds area mass / rest_density to its bin; height = area sum / bin_w (2-D
+ * volume measure). x_centre and height receive n_bins values (size-probe:
+ * PEF_ERR_BOUNDS with *n_bins set when cap is too small). Non-finite bounds or
+ * more than 2^26 bins: PEF_ERR_ARG (the reference would allocate them). */
+int32_t pef_world_column_heights(PefWorld* world, uint32_t scene, double x_min, double x_max, double bin_w,
+                                 double* x_centre, double* height, uint64_t cap, uint64_t* n_bins);
+/* fluid_front_x_height (validation.rs:106-117): the rightmost column-bin centre
+ * (bins of width dx over [-2 dx, channel_length + 2 dx)) whose height is at
+ * least h_frac * h0; -infinity when no bin is wet. */
+int32_t pef_world_front_x_height(PefWorld* world, uint32_t scene, double dx, double h0, double channel_length,
+                                 double h_frac, double* x_front);
+
 /* ---- scene construction (particles.rs:164-347) ---------------------------- */
 
 /* mass_from_spacing (particles.rs:166-171). */

@@ -587,6 +587,23 @@ __global__ void k_probe_candidates(const SceneDev* __restrict__ sc, uint32_t sce
   *out_len = n;
 }
 
+// Fluid particles per column bin of one scene (validation.rs:66-92): bin
+// floor((x - x_min) / bw) over [x_min, x_max), from the FP64 positions.
+__global__ void k_column_counts(uint32_t first, uint32_t count, State cur, double x_min, double x_max, double bw,
+                                uint64_t n_bins, uint32_t* __restrict__ counts) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t s = first + k;
+  if (cur.META[s] & kMetaBoundary) return;
+  const double x = cur.X[s];
+  if (x < x_min || x >= x_max) return;
+  // a NaN position passes both range tests and lands in bin 0, as Rust's
+  // saturating `NaN as usize` puts it there
+  const double q = floor((x - x_min) / bw);
+  const uint64_t b = (q == q) ? (uint64_t)q : 0;
+  if (b < n_bins) atomicAdd(&counts[b], 1u);
+}
+
 // slot of each original index
 __global__ void k_slot_of_id(uint32_t N, State cur, uint32_t* __restrict__ slot_of_id) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -718,6 +735,12 @@ void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene,
                              const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
                              uint64_t* out_len) {
   PEF_LAUNCH(k_probe_candidates, 1, 64, st, sc, scene, key, cell_start, order, out, cap, out_len);
+}
+
+void launch_column_counts(hipStream_t st, uint32_t first, uint32_t count, const State& cur, double x_min,
+                          double x_max, double bw, uint64_t n_bins, uint32_t* counts) {
+  if (count == 0) return;
+  PEF_LAUNCH(k_column_counts, blocks_for(count, 256), 256, st, first, count, cur, x_min, x_max, bw, n_bins, counts);
 }
 
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id) {

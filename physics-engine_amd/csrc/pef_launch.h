@@ -57,6 +57,8 @@ void launch_compact_cells(hipStream_t st, uint32_t C, const SceneDev* sc, uint32
 void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene, const int32_t* key,
                              const uint32_t* cell_start, const uint64_t* order, uint64_t* out, uint64_t cap,
                              uint64_t* out_len);
+void launch_column_counts(hipStream_t st, uint32_t first, uint32_t count, const State& cur, double x_min,
+                          double x_max, double bw, uint64_t n_bins, uint32_t* counts);
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id);
 void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl);
 void launch_calib_copy(hipStream_t st, const float4* src, float4* dst, uint64_t n);

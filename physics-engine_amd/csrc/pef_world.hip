@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -915,6 +916,66 @@ int32_t diagnostics(World& w, uint32_t scene, PefDiagnostics* out) {
   return PEF_OK;
 }
 
+// Volume-based column heights of scene k (validation.rs:66-92): the device
+// counts fluid particles per bin; each bin's height is then summed here the
+// reference's way, one area m / rho0 at a time, so the result is bit-identical.
+int32_t column_heights(World& w, uint32_t scene, double x_min, double x_max, double bin_w, double* xc, double* hgt,
+                       uint64_t cap, uint64_t* n_bins) {
+  if (!n_bins) return PEF_ERR_NULL;
+  if (scene >= w.S) return PEF_ERR_BOUNDS;
+  if (!(std::isfinite(x_min) && std::isfinite(x_max) && std::isfinite(bin_w))) return PEF_ERR_ARG;
+  const double bw = std::max(bin_w, 1e-12);
+  const double nb_d = std::ceil((x_max - x_min) / bw);
+  const uint64_t nb = std::max<uint64_t>(1, nb_d > 0.0 ? (uint64_t)nb_d : 0);
+  if (nb > (1ull << 26)) return PEF_ERR_ARG;
+  *n_bins = nb;
+  if (!xc || !hgt || cap < nb) return PEF_ERR_BOUNDS;
+  int32_t rc = ensure_density(w);
+  if (rc) return rc;
+  uint32_t* d_cnt = nullptr;
+  HIPTRY(hipMalloc(&d_cnt, nb * 4));
+  pef::launch_fill_u32(w.stream, d_cnt, 0u, nb);
+  pef::launch_column_counts(w.stream, (uint32_t)w.scene_first[scene], (uint32_t)w.scene_count[scene], w.cur, x_min,
+                            x_max, bw, nb, d_cnt);
+  std::vector<uint32_t> cnt(nb);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, nb * 4, hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_cnt);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  const PefParams& p = w.params[scene];
+  const double area = p.mass / std::max(p.rest_density, 1e-12);
+  for (uint64_t b = 0; b < nb; ++b) {
+    double sum = 0.0;
+    for (uint32_t k = 0; k < cnt[b]; ++k) sum += area;
+    xc[b] = x_min + ((double)b + 0.5) * bw;
+    hgt[b] = sum / bw;
+  }
+  return PEF_OK;
+}
+
+// fluid_front_x_height (validation.rs:106-117) with front_x_from_height_field
+// (validation.rs:96-104): the rightmost bin centre whose height reaches
+// h_frac * h0, or -infinity when dry.
+int32_t front_x_height(World& w, uint32_t scene, double dx, double h0, double channel_length, double h_frac,
+                       double* out) {
+  if (!out) return PEF_ERR_NULL;
+  const double bw = std::max(dx, 1e-9);
+  const double x_min = -2.0 * dx, x_max = channel_length + 2.0 * dx;
+  uint64_t nb = 0;
+  int32_t rc = column_heights(w, scene, x_min, x_max, bw, nullptr, nullptr, 0, &nb);
+  if (rc != PEF_ERR_BOUNDS && rc != PEF_OK) return rc;
+  std::vector<double> xc(nb), h(nb);
+  rc = column_heights(w, scene, x_min, x_max, bw, xc.data(), h.data(), nb, &nb);
+  if (rc) return rc;
+  const double thr = h_frac * h0;
+  double xf = -std::numeric_limits<double>::infinity();
+  for (uint64_t b = 0; b < nb; ++b)
+    if (h[b] >= thr) xf = xc[b];
+  *out = xf;
+  return PEF_OK;
+}
+
 int32_t set_state(World& w, const double* x, const double* v, const double* psi, const double* n_out) {
   const uint64_t N = w.N;
   for (uint64_t i = 0; i < N; ++i) {
@@ -1332,6 +1393,18 @@ int32_t pef_world_kernel_times(PefWorld* world, const char** names, double* ms, 
     ms[k] = w.prof_ms[k];
   }
   return PEF_OK;
+}
+
+int32_t pef_world_column_heights(PefWorld* world, uint32_t scene, double x_min, double x_max, double bin_w,
+                                 double* x_centre, double* height, uint64_t cap, uint64_t* n_bins) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return column_heights(world->w, scene, x_min, x_max, bin_w, x_centre, height, cap, n_bins); });
+}
+
+int32_t pef_world_front_x_height(PefWorld* world, uint32_t scene, double dx, double h0, double channel_length,
+                                 double h_frac, double* x_front) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return front_x_height(world->w, scene, dx, h0, channel_length, h_frac, x_front); });
 }
 
 int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEntry* table, uint32_t cap,

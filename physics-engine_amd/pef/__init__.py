@@ -531,6 +531,30 @@ class WcsphWorld:
         _check(f(self._h, dt, steps, table, cap, C.byref(n), C.byref(best)), "pef_world_autotune")
         return {"best": best.as_dict(), "trials": [table[i].as_dict() for i in range(n.value)]}
 
+    def column_heights(self, x_min: float, x_max: float, bin_w: float, scene: int = 0):
+        """validation.rs:66-92: (bin centres, heights), fluid area mass / rho0 per particle."""
+        n = C.c_uint64()
+        f = lib().pef_world_column_heights
+        f.restype = C.c_int32
+        f.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                      C.c_uint64, C.POINTER(C.c_uint64)]
+        rc = f(self._h, scene, x_min, x_max, bin_w, None, None, 0, C.byref(n))
+        if rc not in (OK, ERR_BOUNDS):
+            _check(rc, "column_heights")
+        xc, hg = np.zeros(n.value), np.zeros(n.value)
+        _check(f(self._h, scene, x_min, x_max, bin_w, _ptr(xc), _ptr(hg), n.value, C.byref(n)), "column_heights")
+        return xc, hg
+
+    def front_x_height(self, dx: float, h0: float, channel_length: float, h_frac: float = 0.02, scene: int = 0):
+        """validation.rs:106-117."""
+        out = C.c_double()
+        f = lib().pef_world_front_x_height
+        f.restype = C.c_int32
+        f.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+                      C.POINTER(C.c_double)]
+        _check(f(self._h, scene, dx, h0, channel_length, h_frac, C.byref(out)), "front_x_height")
+        return out.value
+
     def tile_info(self) -> dict:
         """Launch configuration of the brick-tiled kernels and brick counts."""
         out = (C.c_uint32 * 23)()
