@@ -280,6 +280,95 @@ int32_t pef_recomputed_akinci_psi(int32_t dim, double h, uint64_t n, const doubl
  * from Xoshiro256StarStar::seed(seed) (crates/pe-math/src/rng.rs:35, 70). */
 int32_t pef_jitter_positions(uint64_t seed, double amp, uint64_t n, const uint8_t* kind, double* x);
 
+/* ---- learned force surrogate ----------------------------------------------
+ * No counterpart in the reference. A per-particle MLP 32 -> 128 -> 128 -> 128
+ * -> 3 with SiLU after the hidden layers, BF16 weights and activations with
+ * FP32 accumulation on MFMA, that predicts a fluid particle's pressure +
+ * viscosity + boundary-pressure acceleration (the a - g of compute_forces,
+ * wcsph.rs:181-280) from 32 features of its neighbourhood:
+ *   [0] rho / rho0 - 1, [1] p / B with B = rho0 c^2 / gamma,
+ *   [2 + 6s .. 4 + 6s] sum over partners in shell s of V_j (x_j - x_i) / h,
+ *   [5 + 6s .. 7 + 6s] sum over partners in shell s of V_j (v_j - v_i) / c,
+ * for the shells s = floor(2.5 q), q = r / h < 2, and V_j the partner's volume
+ * over m / rho0 (rho0 / rho_j for fluid, psi_j rho0 / m for boundary). It is
+ * trained on states of the plain step; the plain force pass is its fallback
+ * and the judge of its error. */
+#define PEF_MLP_FEATURES 32
+#define PEF_MLP_OUTPUTS 3
+#define PEF_MLP_PARAMS 37635
+typedef struct PefMlp PefMlp;
+typedef struct PefDataset PefDataset;
+
+/* Xavier-uniform weights from splitmix64(seed + k), zero biases, identity
+ * normalisation, on the current HIP device. */
+int32_t pef_mlp_create(uint64_t seed, PefMlp** out);
+void pef_mlp_destroy(PefMlp* mlp);
+/* FP32 master parameters, per layer W[in][out] then b: W1 32x128, b1, W2
+ * 128x128, b2, W3 128x128, b3, W4 128x3, b4 (PEF_MLP_PARAMS floats). Setting
+ * them resets the Adam state. */
+int32_t pef_mlp_get_params(PefMlp* mlp, float* out, uint64_t len);
+int32_t pef_mlp_set_params(PefMlp* mlp, const float* params, uint64_t len);
+/* Network input (x - in_mean) * in_inv_std per feature (32 each); predicted
+ * acceleration = network output * out_scale per component (3). */
+int32_t pef_mlp_get_normalization(PefMlp* mlp, float* in_mean, float* in_inv_std, float* out_scale);
+int32_t pef_mlp_set_normalization(PefMlp* mlp, const float* in_mean, const float* in_inv_std,
+                                  const float* out_scale);
+/* features [n][32] -> predicted accelerations [n][3] (host buffers). */
+int32_t pef_mlp_predict(PefMlp* mlp, const float* features, uint64_t n, float* out);
+/* Gradient of the loss mean over n samples and 3 outputs of (y - t / out_scale)^2
+ * for a batch given on the host (targets [n][3] in m/s^2); n a positive multiple
+ * of 256. grads receives PEF_MLP_PARAMS floats. */
+int32_t pef_mlp_gradients(PefMlp* mlp, const float* features, const float* targets, uint64_t n, float* grads,
+                          float* loss);
+/* One bias-corrected Adam step with a caller-given gradient. */
+int32_t pef_mlp_adam_step(PefMlp* mlp, const float* grads, float lr, float beta1, float beta2, float eps);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * samples (a multiple of 256) drawn with replacement by splitmix64 from seed;
+ * loss_history (steps floats, may be NULL) receives each batch's loss. */
+int32_t pef_mlp_train(PefMlp* mlp, PefDataset* ds, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                      float* loss_history);
+
+/* Training samples on the device: features [n][32] and targets [n][3]. */
+int32_t pef_dataset_create(uint64_t capacity, PefDataset** out);
+void pef_dataset_destroy(PefDataset* ds);
+int32_t pef_dataset_len(PefDataset* ds, uint64_t* n);
+int32_t pef_dataset_append(PefDataset* ds, const float* features, const float* targets, uint64_t n);
+int32_t pef_dataset_get(PefDataset* ds, uint64_t first, uint64_t n, float* features, float* targets);
+/* Per-feature mean and 1 / std (0 for a constant feature) and per-component
+ * RMS of the targets, from FP64 sums over the whole set, into the network's
+ * normalisation. */
+int32_t pef_dataset_fit_normalization(PefDataset* ds, PefMlp* mlp);
+
+/* Features of every particle at the current state, [n][32] in original order
+ * (rows of boundary particles zero). */
+int32_t pef_world_get_features(PefWorld* world, float* out, uint64_t len);
+/* Append n samples at the current state, which does not advance: the features
+ * and the plain force's a - g of fluid particles drawn uniformly (with
+ * replacement) by splitmix64 from seed. */
+int32_t pef_world_sample(PefWorld* world, PefDataset* ds, uint64_t n, uint64_t seed);
+
+typedef struct PefSurrogateStats {
+  uint64_t surrogate_steps; /* steps integrated with the predicted acceleration */
+  uint64_t plain_steps;     /* plain steps while attached: checks and steps after a fallback */
+  uint64_t checks;
+  int32_t fallback;         /* 1 once a check failed; every later step is plain */
+  int32_t pad;
+  double last_accel_error;  /* RMS |a_pred - (a - g)| / RMS |a - g| over fluid, last check */
+  double max_accel_error;
+  double last_density_rms;  /* RMS (rho / rho0 - 1) over fluid at the last check */
+  double ref_density_rms;   /* the same at the first check */
+} PefSurrogateStats;
+/* Attach (mlp non-NULL) or detach the surrogate of a single-scene world. Step k
+ * after attaching is a check step when k % check_every == 0: the plain force
+ * integrates it and the prediction for the same state is compared. A check with
+ * an acceleration error above accel_tol, or a density RMS above the first
+ * check's by more than density_tol, latches the fallback. Other steps integrate
+ * the predicted acceleration (boundary particles then get no reaction force).
+ * The world keeps the pointer: keep the network alive while attached. */
+int32_t pef_world_set_surrogate(PefWorld* world, PefMlp* mlp, uint32_t check_every, double accel_tol,
+                                double density_tol);
+int32_t pef_world_surrogate_stats(PefWorld* world, PefSurrogateStats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,10 @@ def lib():
         "ora_world_diagnostics": (None, [vp, d, C.POINTER(OraDiag)]),
         "ora_fnv1a_f64": (u64, [vp, u64]),
         "ora_fnv1a_bytes": (u64, [vp, u64]),
+        "ora_mlp_forward": (None, [vp, vp, vp, u64, i32, vp]),
+        "ora_mlp_grad": (None, [vp, vp, vp, vp, u64, i32, vp, pd]),
+        "ora_adam": (None, [vp, vp, vp, vp, u64, d, d, d, d, C.c_uint32]),
+        "ora_features": (None, [u64, vp, vp, vp, vp, vp, vp, d, d, d, d, d, d, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -359,3 +363,53 @@ class World:
         return dict(total_mass=d.total_mass, momentum=np.array(d.momentum[:]), kinetic_energy=d.kinetic_energy,
                     potential_energy=d.potential_energy, mean_density_error=d.mean_density_error,
                     max_density_error=d.max_density_error, v_max=d.v_max, a_max=d.a_max, hash=d.hash)
+
+
+# --- learned force surrogate (mlp_oracle.cpp) -----------------------------------
+MLP_PARAMS = 37635
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_forward(params, norm, X, bf16=True):
+    """Predicted accelerations [n][3]; norm = in_mean(32) | in_inv_std(32) | out_scale(3)."""
+    params, norm, X = _f64(params), _f64(norm), _f32(X)
+    n = X.shape[0]
+    Y = np.zeros((n, 3))
+    lib().ora_mlp_forward(_p(params), _p(norm), _p(X), n, 1 if bf16 else 0, _p(Y))
+    return Y
+
+
+def mlp_grad(params, norm, X, T, bf16=True):
+    """(gradient [MLP_PARAMS], loss) of the mean squared normalised error."""
+    params, norm, X, T = _f64(params), _f64(norm), _f32(X), _f32(T)
+    g = np.zeros(MLP_PARAMS)
+    loss = C.c_double()
+    lib().ora_mlp_grad(_p(params), _p(norm), _p(X), _p(T), X.shape[0], 1 if bf16 else 0, _p(g), C.byref(loss))
+    return g, loss.value
+
+
+def adam(w, m, v, g, lr, b1, b2, eps, t):
+    """In-place bias-corrected Adam step t on float64 arrays."""
+    for a in (w, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    g = _f64(g)
+    lib().ora_adam(_p(w), _p(m), _p(v), _p(g), w.size, lr, b1, b2, eps, t)
+
+
+def features(x, v, rho, p, kind, psi, rho0, mass, h, c, B, tol=1e-6):
+    """(features [n][32], near-shell-edge flags [n]) by brute force."""
+    x, v, rho, p, psi = _f64(x), _f64(v), _f64(rho), _f64(p), _f64(psi)
+    kind = np.ascontiguousarray(kind, dtype=np.uint8)
+    n = kind.size
+    out = np.zeros((n, 32))
+    edge = np.zeros(n, np.uint8)
+    lib().ora_features(n, _p(x), _p(v), _p(rho), _p(p), _p(kind), _p(psi), rho0, mass, h, c, B, tol, _p(out),
+                       _p(edge))
+    return out, edge.astype(bool)

@@ -1,0 +1,126 @@
+// Learned force surrogate: a per-particle MLP 32 -> 128 -> 128 -> 128 -> 3
+// (SiLU after the three hidden layers) on BF16 MFMA with FP32 accumulation.
+//
+// Parameters are kept as an FP32 master copy (Adam state beside it) and
+// packed to BF16 for the MFMA kernels:
+//   master, in-major W[in][out] per layer, then its bias:
+//     W1 32x128 | b1 128 | W2 128x128 | b2 128 | W3 128x128 | b3 128 | W4 128x3 | b4 3
+//   packed, out-major rows padded for LDS banks (kRowPad BF16 per row):
+//     P1 128 rows x 32 | P2 128 x 128 | P3 128 x 128 | P4 16 x 128 (rows 3..15 zero)
+// Inputs are standardised per feature, (x - mean) * inv_std; the network's
+// output y is scaled back per component, a = y * out_scale.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pef {
+
+constexpr int kMlpIn = 32, kMlpHid = 128, kMlpOut = 3, kMlpOutPad = 16;
+constexpr int kMlpLayers = 4;
+// master parameter offsets (floats)
+constexpr uint32_t kOffW1 = 0, kOffB1 = kOffW1 + kMlpIn * kMlpHid;
+constexpr uint32_t kOffW2 = kOffB1 + kMlpHid, kOffB2 = kOffW2 + kMlpHid * kMlpHid;
+constexpr uint32_t kOffW3 = kOffB2 + kMlpHid, kOffB3 = kOffW3 + kMlpHid * kMlpHid;
+constexpr uint32_t kOffW4 = kOffB3 + kMlpHid, kOffB4 = kOffW4 + kMlpHid * kMlpOut;
+constexpr uint32_t kMlpParams = kOffB4 + kMlpOut;  // 37 635
+// packed BF16 rows: K + kRowPad elements, so 16 consecutive rows start in
+// different LDS banks
+constexpr int kRowPad = 8;
+constexpr int kLd1 = kMlpIn + kRowPad, kLdH = kMlpHid + kRowPad;
+constexpr uint32_t kPk1 = 0, kPk2 = kPk1 + kMlpHid * kLd1, kPk3 = kPk2 + kMlpHid * kLdH, kPk4 = kPk3 + kMlpHid * kLdH;
+constexpr uint32_t kPackedElems = kPk4 + kMlpOutPad * kLdH;
+// FP32 biases as the kernels read them: b1 | b2 | b3 | b4 padded to 16
+constexpr uint32_t kBiasElems = 3 * kMlpHid + kMlpOutPad;
+
+struct MlpDev {
+  const uint16_t* packed;  // kPackedElems BF16
+  const float* bias;       // kBiasElems
+  const float* in_mean;    // kMlpIn
+  const float* in_inv_std; // kMlpIn
+  const float* out_scale;  // kMlpOut
+};
+
+// FP32 master -> packed BF16 + FP32 biases (round to nearest even).
+void launch_mlp_pack(hipStream_t st, const float* master, uint16_t* packed, float* bias);
+// Inference: features [n][32] (row stride fstride floats) -> out [n] float4
+// (xyz = a, w untouched). Rows whose `skip` bit is set in meta (when meta is
+// non-null) are not written.
+void launch_mlp_infer(hipStream_t st, const MlpDev& m, const float* features, uint32_t fstride, uint32_t n,
+                      float4* out, const uint32_t* meta, uint32_t skip_mask, uint32_t n_cu);
+
+// Training scratch for one batch of B samples (B a multiple of 256).
+struct MlpTrainBufs {
+  uint16_t* HT;     // activations, feature-major: H0 32 x B | H1 | H2 | H3 128 x B
+  uint16_t* DT;     // output gradients, feature-major: d1 | d2 | d3 128 x B | d4 16 x B
+  float* partial;   // per chunk: kMlpParams gradient partials
+  float* loss_part; // per 16-sample tile
+  uint32_t B;
+  uint32_t chunks;  // B / kWgradChunk
+};
+constexpr uint32_t kWgradChunk = 256;
+
+// Forward + backward-data of a batch: samples idx[0..B) of the data set
+// (features [.][32], targets [.][3] in physical units), MSE loss over the
+// three normalised outputs, mean over B.
+void launch_mlp_train_fb(hipStream_t st, const MlpDev& m, const float* feat, const float* targ,
+                         const uint32_t* idx, const MlpTrainBufs& tb, uint32_t n_cu);
+// Per-chunk weight and bias gradient partials from HT and DT.
+void launch_mlp_wgrad(hipStream_t st, const MlpTrainBufs& tb);
+// Sum the partials in chunk order into grad (kMlpParams) and the loss.
+void launch_mlp_reduce(hipStream_t st, const MlpTrainBufs& tb, float* grad, float* loss_out);
+// One Adam step on the master parameters (bias-corrected, step t >= 1).
+void launch_mlp_adam(hipStream_t st, float* master, float* m1, float* m2, const float* grad, float lr, float beta1,
+                     float beta2, float eps, uint32_t t);
+// Batch indices: idx[k] = splitmix64(seed + step * B + k) mod n_data.
+void launch_mlp_batch_idx(hipStream_t st, uint32_t* idx, uint32_t B, uint64_t seed, uint64_t step,
+                          uint64_t n_data);
+
+// The network on one device: FP32 master, Adam moments, packed BF16 copy,
+// normalisation (in_mean | in_inv_std | out_scale) and training scratch.
+struct MlpModel {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n_cu = 0;
+  float *master = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *bias = nullptr, *norm = nullptr,
+        *loss = nullptr;
+  uint16_t* packed = nullptr;
+  uint32_t adam_t = 0;  // Adam steps taken
+  MlpTrainBufs tb{};
+  uint32_t* idx = nullptr;
+  void* batch_mem = nullptr;
+  MlpDev dev() const { return MlpDev{packed, bias, norm, norm + kMlpIn, norm + 2 * kMlpIn}; }
+  hipError_t init(uint64_t seed);
+  hipError_t ensure_batch(uint32_t B);  // B a positive multiple of kWgradChunk
+  hipError_t train_step(const float* feat, const float* targ, float lr, float b1, float b2, float eps,
+                        bool apply);  // uses idx[0..B)
+  ~MlpModel();
+};
+
+// Training samples on the device: features [cap][32], targets [cap][3].
+struct DatasetModel {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float* feat = nullptr;
+  float* targ = nullptr;
+  uint64_t cap = 0, len = 0;
+  ~DatasetModel();
+};
+
+// Gather samples of fluid particles: rows F[slot_of_id[id]] and the plain
+// force's a - g of the same slots, for each original id in ids[0..n).
+void launch_sample_gather(hipStream_t st, uint32_t n, const uint32_t* ids, const uint32_t* slot_of_id,
+                          const float* F, const float4* ACC, const uint32_t* META, const void* scenes,
+                          float* out_feat, float* out_targ);
+// Features of every slot into original order (rows of boundary particles zero).
+void launch_unsort_features(hipStream_t st, uint32_t N, const uint32_t* ID, const uint32_t* META, const float* F,
+                            float* out);
+
+}  // namespace pef
+
+struct PefMlp {
+  pef::MlpModel m;
+};
+struct PefDataset {
+  pef::DatasetModel d;
+};

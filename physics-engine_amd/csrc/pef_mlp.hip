@@ -1,0 +1,998 @@
+// BF16 MFMA kernels of the learned force surrogate (inference, training
+// forward/backward, weight-gradient partials, Adam). Layouts: pef_mlp.h.
+//
+// Every matrix product is the gfx950 16x16x32 BF16 MFMA with FP32
+// accumulation. Its lane layout (lane l, r = l % 16, g = l / 16):
+//   A (16 x 32): A[r][8g + j], j = 0..7      (eight BF16 per lane)
+//   B (32 x 16): B[8g + j][r]
+//   D (16 x 16): D[4g + i][r], i = 0..3      (four FP32 per lane)
+// Activations between layers are rounded to BF16 (round to nearest even);
+// weights are BF16 copies of the FP32 master.
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/pef.h"
+#include "pef_common.h"
+#include "pef_mlp.h"
+
+namespace pef {
+
+#define PEF_LAUNCH(kernel, grid, block, stream, ...) \
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ inline bf16x8 frag_ld(const uint16_t* p) {  // 16-byte aligned LDS or global read
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const s16x8*>(p));
+}
+// eight BF16 at stride `ld` elements (a column of an out-major matrix)
+__device__ inline bf16x8 frag_ld_strided(const uint16_t* p, int ld) {
+  s16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (short)p[j * ld];
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ inline bf16x8 frag_pack(const float x[8]) {
+  s16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (short)to_bf16(x[j]);
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x4 splat(float v) { return f32x4{v, v, v, v}; }
+
+__device__ inline float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu_f(float z) { return z * sigmoid_f(z); }
+__device__ inline float dsilu_f(float z) {
+  const float s = sigmoid_f(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// Orders this wave's LDS writes before its later LDS reads across lanes.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// parameter packing
+// ---------------------------------------------------------------------------
+__global__ void k_mlp_pack(const float* __restrict__ m, uint16_t* __restrict__ P, float* __restrict__ bias) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < kPackedElems) {
+    uint32_t layer_off, ld, in, out, rows, base;
+    if (t < kPk2) { base = kPk1; ld = kLd1; in = kMlpIn; out = kMlpHid; rows = kMlpHid; layer_off = kOffW1; }
+    else if (t < kPk3) { base = kPk2; ld = kLdH; in = kMlpHid; out = kMlpHid; rows = kMlpHid; layer_off = kOffW2; }
+    else if (t < kPk4) { base = kPk3; ld = kLdH; in = kMlpHid; out = kMlpHid; rows = kMlpHid; layer_off = kOffW3; }
+    else { base = kPk4; ld = kLdH; in = kMlpHid; out = kMlpOut; rows = kMlpOutPad; layer_off = kOffW4; }
+    const uint32_t e = t - base, o = e / ld, i = e % ld;
+    float v = 0.0f;
+    if (o < out && i < in && o < rows) v = m[layer_off + i * out + o];
+    P[t] = to_bf16(v);
+  }
+  if (t < kBiasElems) {
+    float v;
+    if (t < kMlpHid) v = m[kOffB1 + t];
+    else if (t < 2 * kMlpHid) v = m[kOffB2 + t - kMlpHid];
+    else if (t < 3 * kMlpHid) v = m[kOffB3 + t - 2 * kMlpHid];
+    else v = (t - 3 * kMlpHid < (uint32_t)kMlpOut) ? m[kOffB4 + t - 3 * kMlpHid] : 0.0f;
+    bias[t] = v;
+  }
+}
+
+void launch_mlp_pack(hipStream_t st, const float* master, uint16_t* packed, float* bias) {
+  const uint32_t n = kPackedElems;
+  PEF_LAUNCH(k_mlp_pack, (n + 255) / 256, 256, st, master, packed, bias);
+}
+
+// ---------------------------------------------------------------------------
+// shared forward pieces (one wave, 16 samples)
+// ---------------------------------------------------------------------------
+// Layer 1 from an A fragment of normalised inputs: Z (D layout) per n-block.
+__device__ inline void layer_in(const uint16_t* P, const float* bias, bf16x8 a, int r, int g, f32x4 z[8]) {
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    const bf16x8 b = frag_ld(P + kPk1 + (nb * 16 + r) * kLd1 + 8 * g);
+    z[nb] = mfma(a, b, splat(bias[nb * 16 + r]));
+  }
+}
+// Hidden layer from the wave's LDS activation tile H (16 x 128 BF16).
+__device__ inline void layer_hid(const uint16_t* Pl, const float* bias, const uint16_t* H, int r, int g,
+                                 f32x4 z[8]) {
+  bf16x8 a[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) a[kb] = frag_ld(H + r * kLdH + kb * 32 + 8 * g);
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    f32x4 acc = splat(bias[nb * 16 + r]);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(a[kb], frag_ld(Pl + (nb * 16 + r) * kLdH + kb * 32 + 8 * g), acc);
+    z[nb] = acc;
+  }
+}
+// Output layer (16 padded outputs).
+__device__ inline f32x4 layer_out(const uint16_t* P, const float* bias, const uint16_t* H, int r, int g) {
+  f32x4 acc = splat(bias[3 * kMlpHid + r]);
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+    acc = mfma(frag_ld(H + r * kLdH + kb * 32 + 8 * g), frag_ld(P + kPk4 + r * kLdH + kb * 32 + 8 * g), acc);
+  return acc;
+}
+// SiLU of Z into the LDS tile (row = sample), BF16.
+__device__ inline void store_act(uint16_t* H, const f32x4 z[8], int r, int g) {
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) H[(4 * g + i) * kLdH + nb * 16 + r] = to_bf16(silu_f(z[nb][i]));
+}
+// Normalised input fragment: sample row `row` of F, features 8g..8g+7.
+// mean / inv_std hold this lane's eight features.
+__device__ inline bf16x8 input_frag(const float* F, uint64_t row, uint32_t fstride, int g, const float mean[8],
+                                    const float inv_std[8], float x[8]) {
+  const float4* fp = reinterpret_cast<const float4*>(F + row * fstride + 8 * g);
+  const float4 u = fp[0], v = fp[1];
+  const float raw[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = (raw[j] - mean[j]) * inv_std[j];
+  return frag_pack(x);
+}
+
+constexpr int kInferThreads = 1024;
+constexpr size_t kInferLds = kPackedElems * 2 + kBiasElems * 4 + (kInferThreads / 64) * 16 * kLdH * 2;
+
+__global__ void __launch_bounds__(kInferThreads)
+    k_mlp_infer(MlpDev m, const float* __restrict__ F, uint32_t fstride, uint32_t n, float4* __restrict__ out,
+                const uint32_t* __restrict__ meta, uint32_t skip_mask) {
+  extern __shared__ uint4 smem4[];
+  uint16_t* P = reinterpret_cast<uint16_t*>(smem4);
+  float* bias = reinterpret_cast<float*>(P + kPackedElems);
+  uint16_t* Hall = reinterpret_cast<uint16_t*>(bias + kBiasElems);
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(m.packed);
+    for (uint32_t k = threadIdx.x; k < kPackedElems / 8; k += blockDim.x) smem4[k] = src[k];
+    for (uint32_t k = threadIdx.x; k < kBiasElems; k += blockDim.x) bias[k] = m.bias[k];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  uint16_t* H = Hall + wave * 16 * kLdH;
+  float mean[8], istd[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    mean[j] = m.in_mean[8 * g + j];
+    istd[j] = m.in_inv_std[8 * g + j];
+  }
+  const float osc = (r < kMlpOut) ? m.out_scale[r] : 0.0f;
+  const uint32_t n_tiles = (n + 15) / 16, waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + wave; tile < n_tiles; tile += waves) {
+    const uint32_t row = min(tile * 16 + r, n - 1);
+    float x[8];
+    f32x4 z[8];
+    layer_in(P, bias, input_frag(F, row, fstride, g, mean, istd, x), r, g, z);
+    store_act(H, z, r, g);
+    wave_sync();
+    layer_hid(P + kPk2, bias + kMlpHid, H, r, g, z);
+    wave_sync();
+    store_act(H, z, r, g);
+    wave_sync();
+    layer_hid(P + kPk3, bias + 2 * kMlpHid, H, r, g, z);
+    wave_sync();
+    store_act(H, z, r, g);
+    wave_sync();
+    const f32x4 y = layer_out(P, bias, H, r, g);
+    wave_sync();
+    if (r < kMlpOut) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t s = tile * 16 + 4 * g + i;
+        if (s < n && !(meta && (meta[s] & skip_mask))) reinterpret_cast<float*>(out)[4ull * s + r] = y[i] * osc;
+      }
+    }
+  }
+}
+
+void launch_mlp_infer(hipStream_t st, const MlpDev& m, const float* features, uint32_t fstride, uint32_t n,
+                      float4* out, const uint32_t* meta, uint32_t skip_mask, uint32_t n_cu) {
+  if (n == 0) return;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kInferLds);
+    attr = true;
+  }
+  const uint32_t tiles = (n + 15) / 16, per = kInferThreads / 64;
+  const uint32_t grid = std::max(1u, std::min(n_cu, (tiles + per - 1) / per));
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferThreads), kInferLds, st, m, features, fstride, n, out,
+                     meta, skip_mask);
+}
+
+// ---------------------------------------------------------------------------
+// training: forward + backward-data per 16-sample tile
+// ---------------------------------------------------------------------------
+constexpr int kTrainThreads = 256;
+constexpr size_t kTrainLds = kPackedElems * 2 + kBiasElems * 4 + (kTrainThreads / 64) * 16 * kLdH * 2;
+
+// Store a D-layout block (4 consecutive samples of one feature) feature-major.
+__device__ inline void store_fm(uint16_t* T, uint32_t B, uint32_t feat, uint32_t s0, const uint16_t v[4]) {
+  const uint64_t bits = (uint64_t)v[0] | ((uint64_t)v[1] << 16) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 48);
+  *reinterpret_cast<uint64_t*>(T + (uint64_t)feat * B + s0) = bits;
+}
+
+// delta_{l-1} = (delta_l . W_l^T) (.) SiLU'(Z_{l-1}) for hidden W_l (128 x 128):
+// A = delta_l tile in LDS (16 x 128), B[k = o][n = i] = W_l[i][o] = P_l[o][i].
+__device__ inline void back_hid(const uint16_t* Pl, const uint16_t* D, int r, int g, const f32x4 zprev[8],
+                                f32x4 out[8]) {
+  bf16x8 a[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) a[kb] = frag_ld(D + r * kLdH + kb * 32 + 8 * g);
+#pragma unroll
+  for (int nb = 0; nb < 8; ++nb) {
+    f32x4 acc = splat(0.0f);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      acc = mfma(a[kb], frag_ld_strided(Pl + (kb * 32 + 8 * g) * kLdH + nb * 16 + r, kLdH), acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[nb][i] = acc[i] * dsilu_f(zprev[nb][i]);
+  }
+}
+
+__global__ void __launch_bounds__(kTrainThreads)
+    k_mlp_train_fb(MlpDev m, const float* __restrict__ feat, const float* __restrict__ targ,
+                   const uint32_t* __restrict__ idx, MlpTrainBufs tb) {
+  extern __shared__ uint4 smem4[];
+  uint16_t* P = reinterpret_cast<uint16_t*>(smem4);
+  float* bias = reinterpret_cast<float*>(P + kPackedElems);
+  uint16_t* Hall = reinterpret_cast<uint16_t*>(bias + kBiasElems);
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(m.packed);
+    for (uint32_t k = threadIdx.x; k < kPackedElems / 8; k += blockDim.x) smem4[k] = src[k];
+    for (uint32_t k = threadIdx.x; k < kBiasElems; k += blockDim.x) bias[k] = m.bias[k];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  uint16_t* H = Hall + wave * 16 * kLdH;
+  float mean[8], istd[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    mean[j] = m.in_mean[8 * g + j];
+    istd[j] = m.in_inv_std[8 * g + j];
+  }
+  const uint32_t B = tb.B;
+  uint16_t* H0 = tb.HT;
+  uint16_t* H1 = H0 + (uint64_t)kMlpIn * B;
+  uint16_t* H2 = H1 + (uint64_t)kMlpHid * B;
+  uint16_t* H3 = H2 + (uint64_t)kMlpHid * B;
+  uint16_t* D1 = tb.DT;
+  uint16_t* D2 = D1 + (uint64_t)kMlpHid * B;
+  uint16_t* D3 = D2 + (uint64_t)kMlpHid * B;
+  uint16_t* D4 = D3 + (uint64_t)kMlpHid * B;
+  const float inv_n = 1.0f / (3.0f * (float)B);
+  const float osc = (r < kMlpOut) ? m.out_scale[r] : 1.0f;
+  const uint32_t n_tiles = B / 16, waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + wave; tile < n_tiles; tile += waves) {
+    const uint32_t s_base = tile * 16;
+    // ---- forward, keeping Z1..Z3 in registers
+    float x[8];
+    const bf16x8 a0 = input_frag(feat, idx[s_base + r], kMlpIn, g, mean, istd, x);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) H0[(uint64_t)(8 * g + j) * B + s_base + r] = to_bf16(x[j]);
+    f32x4 z1[8], z2[8], z3[8];
+    uint16_t v4[4];
+    layer_in(P, bias, a0, r, g, z1);
+    store_act(H, z1, r, g);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v4[i] = to_bf16(silu_f(z1[nb][i]));
+      store_fm(H1, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+    wave_sync();
+    layer_hid(P + kPk2, bias + kMlpHid, H, r, g, z2);
+    wave_sync();
+    store_act(H, z2, r, g);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v4[i] = to_bf16(silu_f(z2[nb][i]));
+      store_fm(H2, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+    wave_sync();
+    layer_hid(P + kPk3, bias + 2 * kMlpHid, H, r, g, z3);
+    wave_sync();
+    store_act(H, z3, r, g);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v4[i] = to_bf16(silu_f(z3[nb][i]));
+      store_fm(H3, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+    wave_sync();
+    const f32x4 y = layer_out(P, bias, H, r, g);
+    wave_sync();
+    // ---- loss and output gradient (normalised targets t / out_scale)
+    f32x4 d4;
+    float lsum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float d = 0.0f;
+      if (r < kMlpOut) {
+        const float t = targ[3ull * idx[s_base + 4 * g + i] + r] / osc;
+        d = y[i] - t;
+        lsum += d * d;
+      }
+      d4[i] = 2.0f * d * inv_n;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v4[i] = to_bf16(d4[i]);
+    store_fm(D4, B, r, s_base + 4 * g, v4);
+    // delta_4 as a 16 x 32 BF16 A tile (columns 16..31 zero)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      H[(4 * g + i) * kLdH + r] = v4[i];
+      H[(4 * g + i) * kLdH + 16 + r] = 0;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (lane == 0) tb.loss_part[tile] = lsum;
+    wave_sync();
+    // ---- delta_3 = (delta_4 . W4^T) (.) SiLU'(Z3): B[k = o][n = i] = P4[o][i], o < 16
+    f32x4 d[8];
+    {
+      const bf16x8 a = frag_ld(H + r * kLdH + 8 * g);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        bf16x8 b;
+        if (g < 2) {
+          b = frag_ld_strided(P + kPk4 + (8 * g) * kLdH + nb * 16 + r, kLdH);
+        } else {
+          const s16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+          b = __builtin_bit_cast(bf16x8, zero);
+        }
+        const f32x4 acc = mfma(a, b, splat(0.0f));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[nb][i] = acc[i] * dsilu_f(z3[nb][i]);
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v4[i] = to_bf16(d[nb][i]);
+        H[(4 * g + i) * kLdH + nb * 16 + r] = v4[i];
+      }
+      store_fm(D3, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+    wave_sync();
+    // ---- delta_2, delta_1
+    back_hid(P + kPk3, H, r, g, z2, d);
+    wave_sync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v4[i] = to_bf16(d[nb][i]);
+        H[(4 * g + i) * kLdH + nb * 16 + r] = v4[i];
+      }
+      store_fm(D2, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+    wave_sync();
+    back_hid(P + kPk2, H, r, g, z1, d);
+    wave_sync();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v4[i] = to_bf16(d[nb][i]);
+      store_fm(D1, B, nb * 16 + r, s_base + 4 * g, v4);
+    }
+  }
+}
+
+void launch_mlp_train_fb(hipStream_t st, const MlpDev& m, const float* feat, const float* targ,
+                         const uint32_t* idx, const MlpTrainBufs& tb, uint32_t n_cu) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_mlp_train_fb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kTrainLds);
+    attr = true;
+  }
+  const uint32_t tiles = tb.B / 16, per = kTrainThreads / 64;
+  const uint32_t grid = std::max(1u, std::min(n_cu, (tiles + per - 1) / per));
+  hipLaunchKernelGGL(k_mlp_train_fb, dim3(grid), dim3(kTrainThreads), kTrainLds, st, m, feat, targ, idx, tb);
+}
+
+// ---------------------------------------------------------------------------
+// weight-gradient partials: G_l[i][o] = sum_s H_{l-1}[i][s] delta_l[o][s] over
+// one chunk of kWgradChunk samples; bias gradients sum delta_l over the chunk
+// ---------------------------------------------------------------------------
+struct WgLayer {
+  uint32_t in, out, ib, ob, off_w, off_b;
+};
+__device__ inline WgLayer wg_layer(int l) {
+  switch (l) {
+    case 0: return {kMlpIn, kMlpHid, kMlpIn / 16, kMlpHid / 16, kOffW1, kOffB1};
+    case 1: return {kMlpHid, kMlpHid, 8, 8, kOffW2, kOffB2};
+    case 2: return {kMlpHid, kMlpHid, 8, 8, kOffW3, kOffB3};
+    default: return {kMlpHid, kMlpOut, 8, 1, kOffW4, kOffB4};
+  }
+}
+constexpr uint32_t kWgBlocks = 2 * 8 + 64 + 64 + 8;
+
+__global__ void __launch_bounds__(256) k_mlp_wgrad(MlpTrainBufs tb) {
+  const uint32_t c = blockIdx.x, B = tb.B, s0 = c * kWgradChunk;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  float* part = tb.partial + (uint64_t)c * kMlpParams;
+  const uint16_t* Hs[4] = {tb.HT, tb.HT + (uint64_t)kMlpIn * B, tb.HT + (uint64_t)(kMlpIn + kMlpHid) * B,
+                           tb.HT + (uint64_t)(kMlpIn + 2 * kMlpHid) * B};
+  const uint16_t* Ds[4] = {tb.DT, tb.DT + (uint64_t)kMlpHid * B, tb.DT + (uint64_t)2 * kMlpHid * B,
+                           tb.DT + (uint64_t)3 * kMlpHid * B};
+  for (uint32_t blk = wave; blk < kWgBlocks; blk += blockDim.x >> 6) {
+    int l;
+    uint32_t k = blk;
+    if (k < 16) l = 0;
+    else if ((k -= 16) < 64) l = 1;
+    else if ((k -= 64) < 64) l = 2;
+    else { k -= 64; l = 3; }
+    const WgLayer L = wg_layer(l);
+    const uint32_t i0 = (k / L.ob) * 16, o0 = (k % L.ob) * 16;
+    const uint16_t* hrow = Hs[l] + (uint64_t)(i0 + r) * B + s0 + 8 * g;
+    const uint16_t* drow = Ds[l] + (uint64_t)(o0 + r) * B + s0 + 8 * g;
+    f32x4 acc = splat(0.0f);
+#pragma unroll
+    for (uint32_t st = 0; st < kWgradChunk; st += 32) acc = mfma(frag_ld(hrow + st), frag_ld(drow + st), acc);
+    const uint32_t o = o0 + r;
+    if (o < L.out) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part[L.off_w + (i0 + 4 * g + i) * L.out + o] = acc[i];
+    }
+  }
+  // bias partials, samples in order
+  for (uint32_t t = threadIdx.x; t < 3 * kMlpHid + kMlpOut; t += blockDim.x) {
+    const int l = (t < kMlpHid) ? 0 : (t < 2 * kMlpHid) ? 1 : (t < 3 * kMlpHid) ? 2 : 3;
+    const uint32_t o = t - l * kMlpHid;
+    const uint16_t* drow = Ds[l] + (uint64_t)o * B + s0;
+    float s = 0.0f;
+    for (uint32_t j = 0; j < kWgradChunk; ++j) s += from_bf16(drow[j]);
+    part[wg_layer(l).off_b + o] = s;
+  }
+}
+
+void launch_mlp_wgrad(hipStream_t st, const MlpTrainBufs& tb) {
+  PEF_LAUNCH(k_mlp_wgrad, tb.chunks, 256, st, tb);
+}
+
+__global__ void k_mlp_reduce(MlpTrainBufs tb, float* __restrict__ grad, float* __restrict__ loss_out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kMlpParams) {
+    float s = 0.0f;
+    for (uint32_t c = 0; c < tb.chunks; ++c) s += tb.partial[(uint64_t)c * kMlpParams + p];
+    grad[p] = s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
+    float s = 0.0f;
+    for (uint32_t t = 0; t < tb.B / 16; ++t) s += tb.loss_part[t];
+    *loss_out = s / (3.0f * (float)tb.B);
+  }
+}
+
+void launch_mlp_reduce(hipStream_t st, const MlpTrainBufs& tb, float* grad, float* loss_out) {
+  PEF_LAUNCH(k_mlp_reduce, (kMlpParams + 255) / 256, 256, st, tb, grad, loss_out);
+}
+
+__global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
+                           const float* __restrict__ grad, float lr, float b1, float b2, float eps, float c1,
+                           float c2) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float gr = grad[p];
+  const float a = b1 * m1[p] + (1.0f - b1) * gr;
+  const float v = b2 * m2[p] + (1.0f - b2) * gr * gr;
+  m1[p] = a;
+  m2[p] = v;
+  w[p] -= lr * (a * c1) / (sqrtf(v * c2) + eps);
+}
+
+void launch_mlp_adam(hipStream_t st, float* master, float* m1, float* m2, const float* grad, float lr, float beta1,
+                     float beta2, float eps, uint32_t t) {
+  const double c1 = 1.0 / (1.0 - std::pow((double)beta1, (double)t));
+  const double c2 = 1.0 / (1.0 - std::pow((double)beta2, (double)t));
+  PEF_LAUNCH(k_mlp_adam, (kMlpParams + 255) / 256, 256, st, master, m1, m2, grad, lr, beta1, beta2, eps, (float)c1,
+             (float)c2);
+}
+
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void k_mlp_batch_idx(uint32_t* idx, uint32_t B, uint64_t seed, uint64_t step, uint64_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < B) idx[k] = (uint32_t)(splitmix64(seed + step * B + k) % n);
+}
+void launch_mlp_batch_idx(hipStream_t st, uint32_t* idx, uint32_t B, uint64_t seed, uint64_t step,
+                          uint64_t n_data) {
+  PEF_LAUNCH(k_mlp_batch_idx, (B + 255) / 256, 256, st, idx, B, seed, step, n_data);
+}
+
+// ---------------------------------------------------------------------------
+// data set helpers
+// ---------------------------------------------------------------------------
+__global__ void k_sample_gather(uint32_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ slot_of_id,
+                                const float* __restrict__ F, const float4* __restrict__ ACC,
+                                const uint32_t* __restrict__ META, const SceneDev* __restrict__ sc,
+                                float* __restrict__ of, float* __restrict__ ot) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s = slot_of_id[ids[k]];
+  const SceneDev& S = sc[scene_of(META[s])];
+  const float4* src = reinterpret_cast<const float4*>(F + (uint64_t)s * kMlpIn);
+  float4* dst = reinterpret_cast<float4*>(of + (uint64_t)k * kMlpIn);
+#pragma unroll
+  for (int j = 0; j < kMlpIn / 4; ++j) dst[j] = src[j];
+  const float4 a = ACC[s];
+  ot[3ull * k + 0] = a.x - S.g[0];
+  ot[3ull * k + 1] = a.y - S.g[1];
+  ot[3ull * k + 2] = a.z - S.g[2];
+}
+
+void launch_sample_gather(hipStream_t st, uint32_t n, const uint32_t* ids, const uint32_t* slot_of_id,
+                          const float* F, const float4* ACC, const uint32_t* META, const void* scenes,
+                          float* out_feat, float* out_targ) {
+  if (n == 0) return;
+  PEF_LAUNCH(k_sample_gather, (n + 255) / 256, 256, st, n, ids, slot_of_id, F, ACC, META,
+             reinterpret_cast<const SceneDev*>(scenes), out_feat, out_targ);
+}
+
+__global__ void k_unsort_features(uint32_t N, const uint32_t* __restrict__ ID, const uint32_t* __restrict__ META,
+                                  const float* __restrict__ F, float* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const bool bnd = (META[s] & kMetaBoundary) != 0u;
+  const float4* src = reinterpret_cast<const float4*>(F + (uint64_t)s * kMlpIn);
+  float4* dst = reinterpret_cast<float4*>(out + (uint64_t)ID[s] * kMlpIn);
+#pragma unroll
+  for (int j = 0; j < kMlpIn / 4; ++j) dst[j] = bnd ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : src[j];
+}
+
+void launch_unsort_features(hipStream_t st, uint32_t N, const uint32_t* ID, const uint32_t* META, const float* F,
+                            float* out) {
+  if (N == 0) return;
+  PEF_LAUNCH(k_unsort_features, (N + 255) / 256, 256, st, N, ID, META, F, out);
+}
+
+// FP64 per-block sums of x, x^2 (32 features) and t^2 (3 targets).
+constexpr int kStatVals = 2 * kMlpIn + kMlpOut;
+__global__ void k_ds_stats(uint64_t n, const float* __restrict__ F, const float* __restrict__ T,
+                           double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc[kStatVals];
+  for (int k = 0; k < kStatVals; ++k) acc[k] = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    for (int k = 0; k < kMlpIn; ++k) {
+      const double x = F[i * kMlpIn + k];
+      acc[k] += x;
+      acc[kMlpIn + k] += x * x;
+    }
+    for (int k = 0; k < kMlpOut; ++k) {
+      const double t = T[i * kMlpOut + k];
+      acc[2 * kMlpIn + k] += t * t;
+    }
+  }
+  for (int k = 0; k < kStatVals; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(uint64_t)blockIdx.x * kStatVals + k] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host: model and data set
+// ---------------------------------------------------------------------------
+static uint64_t splitmix64_h(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+#define MLPTRY(x)                  \
+  do {                             \
+    hipError_t e_ = (x);           \
+    if (e_ != hipSuccess) return e_; \
+  } while (0)
+
+hipError_t MlpModel::init(uint64_t seed) {
+  MLPTRY(hipGetDevice(&device));
+  int cu = 0;
+  MLPTRY(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+  n_cu = (uint32_t)cu;
+  MLPTRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  MLPTRY(hipMalloc(&master, kMlpParams * 4));
+  MLPTRY(hipMalloc(&m1, kMlpParams * 4));
+  MLPTRY(hipMalloc(&m2, kMlpParams * 4));
+  MLPTRY(hipMalloc(&grad, kMlpParams * 4));
+  MLPTRY(hipMalloc(&bias, kBiasElems * 4));
+  MLPTRY(hipMalloc(&norm, (2 * kMlpIn + kMlpOut) * 4));
+  MLPTRY(hipMalloc(&loss, 4));
+  MLPTRY(hipMalloc(&packed, kPackedElems * 2));
+  // Xavier-uniform weights from splitmix64(seed + k), zero biases
+  std::vector<float> w(kMlpParams, 0.0f);
+  const uint32_t offw[4] = {kOffW1, kOffW2, kOffW3, kOffW4};
+  const uint32_t fin[4] = {kMlpIn, kMlpHid, kMlpHid, kMlpHid}, fout[4] = {kMlpHid, kMlpHid, kMlpHid, kMlpOut};
+  uint64_t k = 0;
+  for (int l = 0; l < 4; ++l) {
+    const double a = std::sqrt(6.0 / (double)(fin[l] + fout[l]));
+    for (uint32_t j = 0; j < fin[l] * fout[l]; ++j) {
+      const double u = (double)(splitmix64_h(seed + k++) >> 11) * (1.0 / 9007199254740992.0);
+      w[offw[l] + j] = (float)((2.0 * u - 1.0) * a);
+    }
+  }
+  std::vector<float> nm(2 * kMlpIn + kMlpOut, 0.0f);
+  for (int j = kMlpIn; j < 2 * kMlpIn + kMlpOut; ++j) nm[j] = 1.0f;
+  MLPTRY(hipMemcpyAsync(master, w.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+  MLPTRY(hipMemcpyAsync(norm, nm.data(), nm.size() * 4, hipMemcpyHostToDevice, stream));
+  MLPTRY(hipMemsetAsync(m1, 0, kMlpParams * 4, stream));
+  MLPTRY(hipMemsetAsync(m2, 0, kMlpParams * 4, stream));
+  launch_mlp_pack(stream, master, packed, bias);
+  MLPTRY(hipGetLastError());
+  return hipStreamSynchronize(stream);
+}
+
+hipError_t MlpModel::ensure_batch(uint32_t B) {
+  if (tb.B == B && batch_mem) return hipSuccess;
+  if (batch_mem) MLPTRY(hipFree(batch_mem));
+  batch_mem = nullptr;
+  const uint32_t chunks = B / kWgradChunk;
+  const size_t ht = (size_t)(kMlpIn + 3 * kMlpHid) * B * 2, dt = (size_t)(3 * kMlpHid + kMlpOutPad) * B * 2;
+  const size_t part = (size_t)chunks * kMlpParams * 4, lp = (size_t)(B / 16) * 4, ix = (size_t)B * 4;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  MLPTRY(hipMalloc(&batch_mem, up(ht) + up(dt) + up(part) + up(lp) + up(ix)));
+  char* p = static_cast<char*>(batch_mem);
+  tb.HT = reinterpret_cast<uint16_t*>(p);
+  p += up(ht);
+  tb.DT = reinterpret_cast<uint16_t*>(p);
+  p += up(dt);
+  tb.partial = reinterpret_cast<float*>(p);
+  p += up(part);
+  tb.loss_part = reinterpret_cast<float*>(p);
+  p += up(lp);
+  idx = reinterpret_cast<uint32_t*>(p);
+  tb.B = B;
+  tb.chunks = chunks;
+  return hipSuccess;
+}
+
+hipError_t MlpModel::train_step(const float* feat, const float* targ, float lr, float b1, float b2, float eps,
+                                bool apply) {
+  launch_mlp_train_fb(stream, dev(), feat, targ, idx, tb, n_cu);
+  launch_mlp_wgrad(stream, tb);
+  launch_mlp_reduce(stream, tb, grad, loss);
+  if (apply) {
+    launch_mlp_adam(stream, master, m1, m2, grad, lr, b1, b2, eps, ++adam_t);
+    launch_mlp_pack(stream, master, packed, bias);
+  }
+  return hipGetLastError();
+}
+
+MlpModel::~MlpModel() {
+  void* ps[] = {master, m1, m2, grad, bias, norm, loss, packed, batch_mem};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+DatasetModel::~DatasetModel() {
+  if (feat) (void)hipFree(feat);
+  if (targ) (void)hipFree(targ);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace pef
+
+// ===========================================================================
+// C ABI (pef.h): network and data set
+// ===========================================================================
+namespace {
+template <class F>
+int32_t guarded(F&& f) {
+  try {
+    return f();
+  } catch (...) {
+    return PEF_ERR_PANIC;
+  }
+}
+bool finite_all(const float* p, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+}  // namespace
+
+#define ABI_HIP(x)                                  \
+  do {                                              \
+    if ((x) != hipSuccess) return PEF_ERR_PANIC;    \
+  } while (0)
+
+extern "C" {
+
+int32_t pef_mlp_create(uint64_t seed, PefMlp** out) {
+  if (!out) return PEF_ERR_NULL;
+  *out = nullptr;
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return PEF_ERR_DEVICE;
+    PefMlp* m = new PefMlp();
+    if (m->m.init(seed) != hipSuccess) {
+      delete m;
+      return PEF_ERR_PANIC;
+    }
+    *out = m;
+    return PEF_OK;
+  });
+}
+
+void pef_mlp_destroy(PefMlp* mlp) { delete mlp; }
+
+int32_t pef_mlp_get_params(PefMlp* mlp, float* out, uint64_t len) {
+  if (!mlp || !out) return PEF_ERR_NULL;
+  if (len < pef::kMlpParams) return PEF_ERR_BOUNDS;
+  auto& m = mlp->m;
+  ABI_HIP(hipMemcpyAsync(out, m.master, pef::kMlpParams * 4, hipMemcpyDeviceToHost, m.stream));
+  ABI_HIP(hipStreamSynchronize(m.stream));
+  return PEF_OK;
+}
+
+int32_t pef_mlp_set_params(PefMlp* mlp, const float* params, uint64_t len) {
+  if (!mlp || !params) return PEF_ERR_NULL;
+  if (len != pef::kMlpParams) return PEF_ERR_BOUNDS;
+  if (!finite_all(params, len)) return PEF_ERR_ARG;
+  auto& m = mlp->m;
+  ABI_HIP(hipMemcpyAsync(m.master, params, pef::kMlpParams * 4, hipMemcpyHostToDevice, m.stream));
+  ABI_HIP(hipMemsetAsync(m.m1, 0, pef::kMlpParams * 4, m.stream));
+  ABI_HIP(hipMemsetAsync(m.m2, 0, pef::kMlpParams * 4, m.stream));
+  m.adam_t = 0;
+  pef::launch_mlp_pack(m.stream, m.master, m.packed, m.bias);
+  ABI_HIP(hipGetLastError());
+  ABI_HIP(hipStreamSynchronize(m.stream));
+  return PEF_OK;
+}
+
+int32_t pef_mlp_get_normalization(PefMlp* mlp, float* in_mean, float* in_inv_std, float* out_scale) {
+  if (!mlp || !in_mean || !in_inv_std || !out_scale) return PEF_ERR_NULL;
+  auto& m = mlp->m;
+  float h[2 * pef::kMlpIn + pef::kMlpOut];
+  ABI_HIP(hipMemcpyAsync(h, m.norm, sizeof h, hipMemcpyDeviceToHost, m.stream));
+  ABI_HIP(hipStreamSynchronize(m.stream));
+  std::copy(h, h + pef::kMlpIn, in_mean);
+  std::copy(h + pef::kMlpIn, h + 2 * pef::kMlpIn, in_inv_std);
+  std::copy(h + 2 * pef::kMlpIn, h + 2 * pef::kMlpIn + pef::kMlpOut, out_scale);
+  return PEF_OK;
+}
+
+int32_t pef_mlp_set_normalization(PefMlp* mlp, const float* in_mean, const float* in_inv_std,
+                                  const float* out_scale) {
+  if (!mlp || !in_mean || !in_inv_std || !out_scale) return PEF_ERR_NULL;
+  if (!finite_all(in_mean, pef::kMlpIn) || !finite_all(in_inv_std, pef::kMlpIn) ||
+      !finite_all(out_scale, pef::kMlpOut))
+    return PEF_ERR_ARG;
+  auto& m = mlp->m;
+  float h[2 * pef::kMlpIn + pef::kMlpOut];
+  std::copy(in_mean, in_mean + pef::kMlpIn, h);
+  std::copy(in_inv_std, in_inv_std + pef::kMlpIn, h + pef::kMlpIn);
+  std::copy(out_scale, out_scale + pef::kMlpOut, h + 2 * pef::kMlpIn);
+  ABI_HIP(hipMemcpyAsync(m.norm, h, sizeof h, hipMemcpyHostToDevice, m.stream));
+  ABI_HIP(hipStreamSynchronize(m.stream));
+  return PEF_OK;
+}
+
+int32_t pef_mlp_predict(PefMlp* mlp, const float* features, uint64_t n, float* out) {
+  if (!mlp || (n && (!features || !out))) return PEF_ERR_NULL;
+  if (n == 0) return PEF_OK;
+  if (n > (1ull << 26)) return PEF_ERR_BOUNDS;
+  return guarded([&] {
+    auto& m = mlp->m;
+    float* df = nullptr;
+    float4* dout = nullptr;
+    if (hipMalloc(&df, n * pef::kMlpIn * 4) != hipSuccess) return PEF_ERR_PANIC;
+    if (hipMalloc(&dout, n * sizeof(float4)) != hipSuccess) {
+      (void)hipFree(df);
+      return PEF_ERR_PANIC;
+    }
+    std::vector<float4> h(n);
+    hipError_t e = hipMemcpyAsync(df, features, n * pef::kMlpIn * 4, hipMemcpyHostToDevice, m.stream);
+    if (e == hipSuccess) {
+      pef::launch_mlp_infer(m.stream, m.dev(), df, pef::kMlpIn, (uint32_t)n, dout, nullptr, 0u, m.n_cu);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, n * sizeof(float4), hipMemcpyDeviceToHost, m.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m.stream);
+    (void)hipFree(df);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return PEF_ERR_PANIC;
+    for (uint64_t i = 0; i < n; ++i) {
+      out[3 * i] = h[i].x;
+      out[3 * i + 1] = h[i].y;
+      out[3 * i + 2] = h[i].z;
+    }
+    return PEF_OK;
+  });
+}
+
+int32_t pef_mlp_gradients(PefMlp* mlp, const float* features, const float* targets, uint64_t n, float* grads,
+                          float* loss) {
+  if (!mlp || !features || !targets || !grads || !loss) return PEF_ERR_NULL;
+  if (n == 0 || n % pef::kWgradChunk != 0 || n > (1ull << 24)) return PEF_ERR_BOUNDS;
+  return guarded([&] {
+    auto& m = mlp->m;
+    if (m.ensure_batch((uint32_t)n) != hipSuccess) return PEF_ERR_PANIC;
+    float *df = nullptr, *dt = nullptr;
+    if (hipMalloc(&df, n * pef::kMlpIn * 4) != hipSuccess) return PEF_ERR_PANIC;
+    if (hipMalloc(&dt, n * pef::kMlpOut * 4) != hipSuccess) {
+      (void)hipFree(df);
+      return PEF_ERR_PANIC;
+    }
+    std::vector<uint32_t> ident(n);
+    for (uint64_t i = 0; i < n; ++i) ident[i] = (uint32_t)i;
+    hipError_t e = hipMemcpyAsync(df, features, n * pef::kMlpIn * 4, hipMemcpyHostToDevice, m.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, targets, n * pef::kMlpOut * 4, hipMemcpyHostToDevice, m.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m.idx, ident.data(), n * 4, hipMemcpyHostToDevice, m.stream);
+    if (e == hipSuccess) e = m.train_step(df, dt, 0.0f, 0.0f, 0.0f, 0.0f, false);
+    if (e == hipSuccess) e = hipMemcpyAsync(grads, m.grad, pef::kMlpParams * 4, hipMemcpyDeviceToHost, m.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, m.loss, 4, hipMemcpyDeviceToHost, m.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m.stream);
+    (void)hipFree(df);
+    (void)hipFree(dt);
+    return e == hipSuccess ? PEF_OK : PEF_ERR_PANIC;
+  });
+}
+
+int32_t pef_mlp_adam_step(PefMlp* mlp, const float* grads, float lr, float beta1, float beta2, float eps) {
+  if (!mlp || !grads) return PEF_ERR_NULL;
+  if (!finite_all(grads, pef::kMlpParams) || !(lr >= 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) ||
+      !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+    return PEF_ERR_ARG;
+  auto& m = mlp->m;
+  ABI_HIP(hipMemcpyAsync(m.grad, grads, pef::kMlpParams * 4, hipMemcpyHostToDevice, m.stream));
+  pef::launch_mlp_adam(m.stream, m.master, m.m1, m.m2, m.grad, lr, beta1, beta2, eps, ++m.adam_t);
+  pef::launch_mlp_pack(m.stream, m.master, m.packed, m.bias);
+  ABI_HIP(hipGetLastError());
+  ABI_HIP(hipStreamSynchronize(m.stream));
+  return PEF_OK;
+}
+
+int32_t pef_mlp_train(PefMlp* mlp, PefDataset* ds, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                      float* loss_history) {
+  if (!mlp || !ds) return PEF_ERR_NULL;
+  if (batch == 0 || batch % pef::kWgradChunk != 0 || batch > (1u << 20)) return PEF_ERR_BOUNDS;
+  if (!(lr > 0.0f) || !std::isfinite(lr)) return PEF_ERR_ARG;
+  if (ds->d.len == 0) return PEF_ERR_BOUNDS;
+  return guarded([&] {
+    auto& m = mlp->m;
+    if (ds->d.device != m.device) return PEF_ERR_ARG;
+    if (m.ensure_batch(batch) != hipSuccess) return PEF_ERR_PANIC;
+    float* dloss = nullptr;
+    if (steps && hipMalloc(&dloss, (size_t)steps * 4) != hipSuccess) return PEF_ERR_PANIC;
+    if (hipStreamSynchronize(ds->d.stream) != hipSuccess) return PEF_ERR_PANIC;
+    hipError_t e = hipSuccess;
+    for (uint32_t k = 0; k < steps && e == hipSuccess; ++k) {
+      pef::launch_mlp_batch_idx(m.stream, m.idx, batch, seed, m.adam_t, ds->d.len);
+      e = m.train_step(ds->d.feat, ds->d.targ, lr, 0.9f, 0.999f, 1e-8f, true);
+      if (e == hipSuccess) e = hipMemcpyAsync(dloss + k, m.loss, 4, hipMemcpyDeviceToDevice, m.stream);
+    }
+    if (e == hipSuccess && loss_history && steps)
+      e = hipMemcpyAsync(loss_history, dloss, (size_t)steps * 4, hipMemcpyDeviceToHost, m.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m.stream);
+    if (dloss) (void)hipFree(dloss);
+    return e == hipSuccess ? PEF_OK : PEF_ERR_PANIC;
+  });
+}
+
+int32_t pef_dataset_create(uint64_t capacity, PefDataset** out) {
+  if (!out) return PEF_ERR_NULL;
+  *out = nullptr;
+  if (capacity == 0 || capacity > (1ull << 28)) return PEF_ERR_BOUNDS;
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return PEF_ERR_DEVICE;
+    PefDataset* d = new PefDataset();
+    auto& D = d->d;
+    hipError_t e = hipGetDevice(&D.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&D.feat, capacity * pef::kMlpIn * 4);
+    if (e == hipSuccess) e = hipMalloc(&D.targ, capacity * pef::kMlpOut * 4);
+    if (e != hipSuccess) {
+      delete d;
+      return PEF_ERR_PANIC;
+    }
+    D.cap = capacity;
+    *out = d;
+    return PEF_OK;
+  });
+}
+
+void pef_dataset_destroy(PefDataset* ds) { delete ds; }
+
+int32_t pef_dataset_len(PefDataset* ds, uint64_t* n) {
+  if (!ds || !n) return PEF_ERR_NULL;
+  *n = ds->d.len;
+  return PEF_OK;
+}
+
+int32_t pef_dataset_append(PefDataset* ds, const float* features, const float* targets, uint64_t n) {
+  if (!ds || (n && (!features || !targets))) return PEF_ERR_NULL;
+  auto& D = ds->d;
+  if (D.len + n > D.cap) return PEF_ERR_BOUNDS;
+  if (!finite_all(features, n * pef::kMlpIn) || !finite_all(targets, n * pef::kMlpOut)) return PEF_ERR_ARG;
+  ABI_HIP(hipMemcpyAsync(D.feat + D.len * pef::kMlpIn, features, n * pef::kMlpIn * 4, hipMemcpyHostToDevice,
+                         D.stream));
+  ABI_HIP(hipMemcpyAsync(D.targ + D.len * pef::kMlpOut, targets, n * pef::kMlpOut * 4, hipMemcpyHostToDevice,
+                         D.stream));
+  ABI_HIP(hipStreamSynchronize(D.stream));
+  D.len += n;
+  return PEF_OK;
+}
+
+int32_t pef_dataset_get(PefDataset* ds, uint64_t first, uint64_t n, float* features, float* targets) {
+  if (!ds || (n && (!features || !targets))) return PEF_ERR_NULL;
+  auto& D = ds->d;
+  if (first > D.len || n > D.len - first) return PEF_ERR_BOUNDS;
+  ABI_HIP(hipMemcpyAsync(features, D.feat + first * pef::kMlpIn, n * pef::kMlpIn * 4, hipMemcpyDeviceToHost,
+                         D.stream));
+  ABI_HIP(hipMemcpyAsync(targets, D.targ + first * pef::kMlpOut, n * pef::kMlpOut * 4, hipMemcpyDeviceToHost,
+                         D.stream));
+  ABI_HIP(hipStreamSynchronize(D.stream));
+  return PEF_OK;
+}
+
+int32_t pef_dataset_fit_normalization(PefDataset* ds, PefMlp* mlp) {
+  if (!ds || !mlp) return PEF_ERR_NULL;
+  auto& D = ds->d;
+  if (D.len == 0) return PEF_ERR_BOUNDS;
+  return guarded([&] {
+    const uint32_t blocks = 256;
+    double* part = nullptr;
+    if (hipMalloc(&part, (size_t)blocks * pef::kStatVals * 8) != hipSuccess) return PEF_ERR_PANIC;
+    std::vector<double> h((size_t)blocks * pef::kStatVals);
+    hipLaunchKernelGGL(pef::k_ds_stats, dim3(blocks), dim3(256), 0, D.stream, D.len, D.feat, D.targ, part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, D.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return PEF_ERR_PANIC;
+    double sum[pef::kStatVals] = {};
+    for (uint32_t b = 0; b < blocks; ++b)
+      for (int k = 0; k < pef::kStatVals; ++k) sum[k] += h[(size_t)b * pef::kStatVals + k];
+    const double n = (double)D.len;
+    float mean[pef::kMlpIn], istd[pef::kMlpIn], osc[pef::kMlpOut];
+    for (int k = 0; k < pef::kMlpIn; ++k) {
+      const double mu = sum[k] / n, var = std::max(sum[pef::kMlpIn + k] / n - mu * mu, 0.0);
+      mean[k] = (float)mu;
+      istd[k] = var > 1e-30 ? (float)(1.0 / std::sqrt(var)) : 0.0f;
+    }
+    for (int k = 0; k < pef::kMlpOut; ++k) osc[k] = (float)std::max(std::sqrt(sum[2 * pef::kMlpIn + k] / n), 1e-12);
+    return pef_mlp_set_normalization(mlp, mean, istd, osc);
+  });
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@
 #include "../../include/pef.h"
 #include "pef_common.h"
 #include "pef_launch.h"
+#include "pef_mlp.h"
 #include "pef_pair.h"
 
 namespace pef {
@@ -86,6 +87,15 @@ class World {
            *probe_flagpos = nullptr, *probe_slot_of_rank = nullptr;
   uint64_t* probe_order = nullptr;
   int32_t* probe_keys = nullptr;
+  // learned force surrogate (allocated on first use)
+  float* FEAT = nullptr;      // [N][32] features, slot order
+  float4* PRED = nullptr;     // predicted a - g, slot order
+  double* sur_part = nullptr; // check-step partial sums
+  PefMlp* sur = nullptr;      // attached network (not owned)
+  uint32_t sur_every = 10;
+  double sur_accel_tol = 0.25, sur_density_tol = 0.005;
+  uint64_t sur_k = 0;         // steps since attaching
+  PefSurrogateStats sur_stats{};
 
   ~World() { release(); }
 
@@ -384,7 +394,7 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   ALLOC(w.brick_sums, (size_t)(w.NB / pef::kScanTileHost) + 2);
   ALLOC(w.bricks, (size_t)w.NB);
   ALLOC(w.brick_tabs, (size_t)w.NB * (pef::kHaloTabBytes / 16));
-  ALLOC(w.queues, 16 * pef::kTileKinds);
+  ALLOC(w.queues, 16 * pef::kTileQueues);
   ALLOC(w.stage, 3ull * N);
 #undef ALLOC
   if (e != hipSuccess) {
@@ -438,7 +448,7 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   if (e == hipSuccess) e = hipMemcpyAsync(w.ctl_d, &hctl, sizeof(hctl), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(w.counts, 0, (size_t)w.C * 4, st);
   if (e == hipSuccess) e = hipMemsetAsync(w.CNT, 0, (size_t)N * 4, st);
-  if (e == hipSuccess) e = hipMemsetAsync(w.queues, 0, 16 * pef::kTileKinds * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(w.queues, 0, 16 * pef::kTileQueues * 4, st);
   if (e == hipSuccess) e = hipMemsetAsync(w.brick_pos, 0, ((size_t)w.NB + 1) * 4, st);
   if (e == hipSuccess) {
     pef::launch_stage(st, N, w.sc_d, w.cur, dx, dv, dk, dpsi, dn, dscene);
@@ -603,6 +613,89 @@ int32_t ensure_density(World& w) {
   return PEF_OK;
 }
 
+int32_t sur_buffers(World& w) {
+  if (w.FEAT) return PEF_OK;
+  HIPTRY(w.dalloc(&w.FEAT, (size_t)w.N * pef::kSurFeatures));
+  HIPTRY(hipMemsetAsync(w.FEAT, 0, (size_t)w.N * pef::kSurFeatures * sizeof(float), w.stream));
+  HIPTRY(w.dalloc(&w.PRED, (size_t)w.N));
+  HIPTRY(hipMemsetAsync(w.PRED, 0, (size_t)w.N * sizeof(float4), w.stream));
+  HIPTRY(w.dalloc(&w.sur_part, (size_t)pef::sur_check_blocks(w.N) * 4));
+  return PEF_OK;
+}
+
+void launch_features(World& w) {
+  pef::launch_features_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.st_d, w.cur, w.FEAT);
+}
+
+// The pressure of the current state when the density pass has not made it.
+void fresh_pressure(World& w) {
+  if (w.any_adapt || w.pressure_stale) {
+    pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+    w.pressure_stale = false;
+  }
+}
+
+// Steps with the surrogate attached, launched eagerly: a check step runs the
+// features, the prediction and the plain force (which integrates) and compares
+// them; other steps integrate the prediction. The rest of the body (rebuild,
+// density, XSPH) is the plain step's.
+int32_t do_step_surrogate(World& w, double dt, uint32_t nsteps) {
+  int32_t rc = sur_buffers(w);
+  if (rc) return rc;
+  const uint32_t nb = pef::sur_check_blocks(w.N);
+  std::vector<double> part((size_t)nb * 4);
+  PefSurrogateStats& st = w.sur_stats;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const bool fallback = st.fallback != 0;
+    const bool check = !fallback && (w.sur_k % w.sur_every == 0);
+    if (w.any_adapt) pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+    if (fallback) {
+      launch_force(w, 1);
+    } else {
+      launch_features(w);
+      pef::launch_mlp_infer(w.stream, w.sur->m.dev(), w.FEAT, pef::kSurFeatures, w.N, w.PRED, w.cur.META,
+                            pef::kMetaBoundary, w.tc.n_cu);
+      if (check) {
+        launch_force(w, 1);
+        pef::launch_sur_check(w.stream, w.N, w.sc_d, w.cur, w.PRED, w.sur_part);
+      } else {
+        pef::launch_integrate_pred(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.cur, w.PRED);
+      }
+    }
+    enqueue_rebuild(w, nullptr);
+    launch_density(w);
+    launch_xsph(w);
+    HIPTRY(hipGetLastError());
+    w.t += dt;
+    ++w.sur_k;
+    if (fallback) {
+      ++st.plain_steps;
+    } else if (!check) {
+      ++st.surrogate_steps;
+    } else {
+      ++st.plain_steps;
+      ++st.checks;
+      HIPTRY(hipMemcpyAsync(part.data(), w.sur_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+      HIPTRY(hipStreamSynchronize(w.stream));
+      double e2 = 0.0, a2 = 0.0, d2 = 0.0, n = 0.0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        e2 += part[4 * b];
+        a2 += part[4 * b + 1];
+        d2 += part[4 * b + 2];
+        n += part[4 * b + 3];
+      }
+      st.last_accel_error = a2 > 0.0 ? std::sqrt(e2 / a2) : 0.0;
+      st.max_accel_error = std::max(st.max_accel_error, st.last_accel_error);
+      st.last_density_rms = n > 0.0 ? std::sqrt(d2 / n) : 0.0;
+      if (st.checks == 1) st.ref_density_rms = st.last_density_rms;
+      if (!(st.last_accel_error <= w.sur_accel_tol) ||
+          !(st.last_density_rms <= st.ref_density_rms + w.sur_density_tol))
+        st.fallback = 1;
+    }
+  }
+  return PEF_OK;
+}
+
 int32_t do_step(World& w, double dt, uint32_t nsteps) {
   if (!(std::isfinite(dt) && dt > 0.0)) return PEF_ERR_ARG;
   if (nsteps == 0) return PEF_OK;
@@ -615,6 +708,11 @@ int32_t do_step(World& w, double dt, uint32_t nsteps) {
     pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
     HIPTRY(hipGetLastError());
     w.pressure_stale = false;
+  }
+  if (w.sur) {
+    rc = do_step_surrogate(w, dt, nsteps);
+    if (rc) return rc;
+    return read_flags(w);
   }
   if (w.profiling || !w.use_graph) {
     for (uint32_t k = 0; k < nsteps; ++k) {
@@ -973,6 +1071,85 @@ int32_t front_x_height(World& w, uint32_t scene, double dx, double h0, double ch
   for (uint64_t b = 0; b < nb; ++b)
     if (h[b] >= thr) xf = xc[b];
   *out = xf;
+  return PEF_OK;
+}
+
+int32_t get_features(World& w, float* out, uint64_t len) {
+  if (!out) return PEF_ERR_NULL;
+  if (len < (uint64_t)w.N * pef::kSurFeatures) return PEF_ERR_BOUNDS;
+  int32_t rc = ensure_density(w);
+  if (!rc) rc = sur_buffers(w);
+  if (rc) return rc;
+  fresh_pressure(w);
+  launch_features(w);
+  float* d = nullptr;
+  HIPTRY(hipMalloc(&d, (size_t)w.N * pef::kSurFeatures * sizeof(float)));
+  pef::launch_unsort_features(w.stream, w.N, w.cur.ID, w.cur.META, w.FEAT, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d, (size_t)w.N * pef::kSurFeatures * sizeof(float), hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d);
+  return e == hipSuccess ? read_flags(w) : PEF_ERR_PANIC;
+}
+
+uint64_t splitmix64_host(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int32_t sample(World& w, PefDataset* ds, uint64_t n, uint64_t seed) {
+  auto& D = ds->d;
+  if (D.device != w.device) return PEF_ERR_ARG;
+  if (n > D.cap - D.len) return PEF_ERR_BOUNDS;
+  if (n == 0) return PEF_OK;
+  std::vector<uint32_t> fluid;
+  for (uint32_t i = 0; i < w.N; ++i)
+    if (w.h_kind[i] == PEF_FLUID) fluid.push_back(i);
+  if (fluid.empty()) return PEF_ERR_ARG;
+  std::vector<uint32_t> ids(n);
+  for (uint64_t k = 0; k < n; ++k) ids[k] = fluid[splitmix64_host(seed + k) % fluid.size()];
+  int32_t rc = ensure_density(w);
+  if (!rc) rc = sur_buffers(w);
+  if (rc) return rc;
+  fresh_pressure(w);
+  launch_features(w);
+  launch_force(w, 0);
+  uint32_t *d_ids = nullptr, *d_slot = nullptr;
+  HIPTRY(hipMalloc(&d_ids, n * 4));
+  hipError_t e = hipMalloc(&d_slot, (size_t)w.N * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_ids, ids.data(), n * 4, hipMemcpyHostToDevice, w.stream);
+  if (e == hipSuccess) {
+    pef::launch_slot_of_id(w.stream, w.N, w.cur, d_slot);
+    pef::launch_sample_gather(w.stream, (uint32_t)n, d_ids, d_slot, w.FEAT, w.cur.ACC, w.cur.META, w.sc_d,
+                              D.feat + D.len * pef::kSurFeatures, D.targ + D.len * 3);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  (void)hipFree(d_ids);
+  if (d_slot) (void)hipFree(d_slot);
+  if (e != hipSuccess) return PEF_ERR_PANIC;
+  D.len += n;
+  return read_flags(w);
+}
+
+int32_t set_surrogate(World& w, PefMlp* mlp, uint32_t check_every, double accel_tol, double density_tol) {
+  if (mlp) {
+    if (w.S != 1) return PEF_ERR_UNSUPPORTED;
+    if (mlp->m.device != w.device) return PEF_ERR_ARG;
+    if (check_every == 0 || !(accel_tol >= 0.0) || !(density_tol >= 0.0)) return PEF_ERR_ARG;
+    int32_t rc = sur_buffers(w);
+    if (rc) return rc;
+    HIPTRY(hipStreamSynchronize(mlp->m.stream));
+  }
+  w.sur = mlp;
+  w.sur_every = check_every ? check_every : 1;
+  w.sur_accel_tol = accel_tol;
+  w.sur_density_tol = density_tol;
+  w.sur_k = 0;
+  w.sur_stats = PefSurrogateStats{};
   return PEF_OK;
 }
 
@@ -1368,6 +1545,28 @@ int32_t pef_world_get_neighbours(PefWorld* world, uint64_t i, uint64_t* buf, uin
 int32_t pef_world_get_neighbour_list(PefWorld* world, uint64_t i, uint64_t* buf, uint64_t cap, uint64_t* out_len) {
   if (!world) return PEF_ERR_NULL;
   return guard([&] { return get_neighbour_list(world->w, i, buf, cap, out_len); });
+}
+
+int32_t pef_world_get_features(PefWorld* world, float* out, uint64_t len) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return get_features(world->w, out, len); });
+}
+
+int32_t pef_world_sample(PefWorld* world, PefDataset* ds, uint64_t n, uint64_t seed) {
+  if (!world || !ds) return PEF_ERR_NULL;
+  return guard([&] { return sample(world->w, ds, n, seed); });
+}
+
+int32_t pef_world_set_surrogate(PefWorld* world, PefMlp* mlp, uint32_t check_every, double accel_tol,
+                                double density_tol) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return set_surrogate(world->w, mlp, check_every, accel_tol, density_tol); });
+}
+
+int32_t pef_world_surrogate_stats(PefWorld* world, PefSurrogateStats* out) {
+  if (!world || !out) return PEF_ERR_NULL;
+  *out = world->w.sur_stats;
+  return PEF_OK;
 }
 
 int32_t pef_world_diagnostics(PefWorld* world, uint32_t scene, PefDiagnostics* out) {

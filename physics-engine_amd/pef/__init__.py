@@ -115,6 +115,13 @@ class PefDiagnostics(C.Structure):
 _lib = None
 
 
+class PefSurrogateStats(C.Structure):
+    _fields_ = [("surrogate_steps", C.c_uint64), ("plain_steps", C.c_uint64), ("checks", C.c_uint64),
+                ("fallback", C.c_int32), ("pad", C.c_int32), ("last_accel_error", C.c_double),
+                ("max_accel_error", C.c_double), ("last_density_rms", C.c_double),
+                ("ref_density_rms", C.c_double)]
+
+
 def lib():
     """Load libpef.so (built in-tree by physics-engine_amd/Makefile)."""
     global _lib
@@ -159,6 +166,26 @@ def lib():
         "pef_fill_box_boundary": (i32, [i32, vp, vp, d, u64, vp, u64, P(u64)]),
         "pef_recomputed_akinci_psi": (i32, [i32, d, u64, vp, vp, vp]),
         "pef_jitter_positions": (i32, [u64, d, u64, vp, vp]),
+        "pef_mlp_create": (i32, [u64, P(vp)]),
+        "pef_mlp_destroy": (None, [vp]),
+        "pef_mlp_get_params": (i32, [vp, vp, u64]),
+        "pef_mlp_set_params": (i32, [vp, vp, u64]),
+        "pef_mlp_get_normalization": (i32, [vp, vp, vp, vp]),
+        "pef_mlp_set_normalization": (i32, [vp, vp, vp, vp]),
+        "pef_mlp_predict": (i32, [vp, vp, u64, vp]),
+        "pef_mlp_gradients": (i32, [vp, vp, vp, u64, vp, P(C.c_float)]),
+        "pef_mlp_adam_step": (i32, [vp, vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "pef_mlp_train": (i32, [vp, vp, u32, u32, C.c_float, u64, vp]),
+        "pef_dataset_create": (i32, [u64, P(vp)]),
+        "pef_dataset_destroy": (None, [vp]),
+        "pef_dataset_len": (i32, [vp, P(u64)]),
+        "pef_dataset_append": (i32, [vp, vp, vp, u64]),
+        "pef_dataset_get": (i32, [vp, u64, u64, vp, vp]),
+        "pef_dataset_fit_normalization": (i32, [vp, vp]),
+        "pef_world_get_features": (i32, [vp, vp, u64]),
+        "pef_world_sample": (i32, [vp, vp, u64, u64]),
+        "pef_world_set_surrogate": (i32, [vp, vp, u32, d, d]),
+        "pef_world_surrogate_stats": (i32, [vp, P(PefSurrogateStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -555,6 +582,28 @@ class WcsphWorld:
         _check(f(self._h, scene, dx, h0, channel_length, h_frac, C.byref(out)), "front_x_height")
         return out.value
 
+    # --- learned force surrogate ---
+    def features(self) -> np.ndarray:
+        """[n][32] surrogate features at the current state, original order."""
+        out = np.zeros((self.n, MLP_FEATURES), np.float32)
+        _check(lib().pef_world_get_features(self._h, _ptr(out), out.size), "get_features")
+        return out
+
+    def sample(self, ds: "Dataset", n: int, seed: int):
+        """Append n (features, a - g) samples of fluid particles at the current state."""
+        _check(lib().pef_world_sample(self._h, ds._h, n, seed), "world_sample")
+
+    def set_surrogate(self, mlp: Optional["Mlp"], check_every: int = 10, accel_tol: float = 0.25,
+                      density_tol: float = 0.005):
+        _check(lib().pef_world_set_surrogate(self._h, mlp._h if mlp is not None else None, check_every, accel_tol,
+                                             density_tol), "set_surrogate")
+        self._sur = mlp  # keeps the network alive while attached
+
+    def surrogate_stats(self) -> dict:
+        st = PefSurrogateStats()
+        _check(lib().pef_world_surrogate_stats(self._h, C.byref(st)), "surrogate_stats")
+        return {k: getattr(st, k) for k, _ in PefSurrogateStats._fields_ if k != "pad"}
+
     def tile_info(self) -> dict:
         """Launch configuration of the brick-tiled kernels and brick counts."""
         out = (C.c_uint32 * 23)()
@@ -568,6 +617,121 @@ class WcsphWorld:
                 "max_lds": v[4], "lds_cap": dict(zip(kinds, v[5:9])), "per_cu": dict(zip(kinds, v[9:13])),
                 "grid": dict(zip(kinds, v[13:17])), "list_capacity": v[17], "bricks": v[18],
                 "nonempty_bricks": v[19], "slow_bricks": v[20], "max_list": v[21]}
+
+
+# --- learned force surrogate ------------------------------------------------------
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 32, 3, 37635
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Mlp:
+    """Per-particle MLP 32 -> 128 -> 128 -> 128 -> 3 on BF16 MFMA (pef_mlp_*)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().pef_mlp_create(seed, C.byref(self._h)), "mlp_create")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().pef_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pef_mlp_get_params(self._h, _ptr(out), out.size), "mlp_get_params")
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        _check(lib().pef_mlp_set_params(self._h, _ptr(p), p.size), "mlp_set_params")
+
+    @property
+    def normalization(self):
+        m, s, o = np.zeros(32, np.float32), np.zeros(32, np.float32), np.zeros(3, np.float32)
+        _check(lib().pef_mlp_get_normalization(self._h, _ptr(m), _ptr(s), _ptr(o)), "mlp_get_normalization")
+        return m, s, o
+
+    def set_normalization(self, in_mean, in_inv_std, out_scale):
+        m, s, o = _f32(in_mean), _f32(in_inv_std), _f32(out_scale)
+        _check(lib().pef_mlp_set_normalization(self._h, _ptr(m), _ptr(s), _ptr(o)), "mlp_set_normalization")
+
+    def predict(self, features) -> np.ndarray:
+        X = _f32(features, (-1, MLP_FEATURES))
+        out = np.zeros((X.shape[0], 3), np.float32)
+        _check(lib().pef_mlp_predict(self._h, _ptr(X), X.shape[0], _ptr(out)), "mlp_predict")
+        return out
+
+    def gradients(self, features, targets):
+        X, T = _f32(features, (-1, MLP_FEATURES)), _f32(targets, (-1, 3))
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().pef_mlp_gradients(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(g), C.byref(loss)),
+               "mlp_gradients")
+        return g, loss.value
+
+    def adam_step(self, grads, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        g = _f32(grads)
+        _check(lib().pef_mlp_adam_step(self._h, _ptr(g), lr, beta1, beta2, eps), "mlp_adam_step")
+
+    def train(self, ds: "Dataset", steps: int, batch: int = 65536, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        hist = np.zeros(steps, np.float32)
+        _check(lib().pef_mlp_train(self._h, ds._h, steps, batch, lr, seed, _ptr(hist) if steps else None),
+               "mlp_train")
+        return hist
+
+    def save(self, path: str):
+        m, s, o = self.normalization
+        np.savez(path, params=self.params, in_mean=m, in_inv_std=s, out_scale=o)
+
+    def load(self, path: str):
+        z = np.load(path, allow_pickle=False)
+        self.params = z["params"]
+        self.set_normalization(z["in_mean"], z["in_inv_std"], z["out_scale"])
+
+
+class Dataset:
+    """Training samples on the device (pef_dataset_*)."""
+
+    def __init__(self, capacity: int):
+        self._h = C.c_void_p()
+        _check(lib().pef_dataset_create(capacity, C.byref(self._h)), "dataset_create")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().pef_dataset_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __len__(self):
+        n = C.c_uint64()
+        _check(lib().pef_dataset_len(self._h, C.byref(n)), "dataset_len")
+        return n.value
+
+    def append(self, features, targets):
+        X, T = _f32(features, (-1, MLP_FEATURES)), _f32(targets, (-1, 3))
+        _check(lib().pef_dataset_append(self._h, _ptr(X), _ptr(T), X.shape[0]), "dataset_append")
+
+    def get(self, first: int = 0, n: Optional[int] = None):
+        n = len(self) - first if n is None else n
+        X, T = np.zeros((n, MLP_FEATURES), np.float32), np.zeros((n, 3), np.float32)
+        _check(lib().pef_dataset_get(self._h, first, n, _ptr(X), _ptr(T)), "dataset_get")
+        return X, T
+
+    def fit_normalization(self, mlp: Mlp):
+        _check(lib().pef_dataset_fit_normalization(self._h, mlp._h), "dataset_fit_normalization")
 
 
 def version() -> str:

@@ -1,0 +1,111 @@
+"""Pins of the surrogate's CPU oracle (oracle/mlp_oracle.cpp): the analytic
+gradient against central finite differences, Adam against its closed form,
+and the features against closed forms of simple neighbourhoods."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+P = O.MLP_PARAMS
+OFF = dict(W1=0, b1=4096, W2=4224, b2=20608, W3=20736, b3=37120, W4=37248, b4=37632)
+
+
+def _net(seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(P)
+    for w, (fi, fo) in (("W1", (32, 128)), ("W2", (128, 128)), ("W3", (128, 128)), ("W4", (128, 3))):
+        a = np.sqrt(6.0 / (fi + fo))
+        p[OFF[w]:OFF[w] + fi * fo] = rng.uniform(-a, a, fi * fo) * scale
+    for b, n in (("b1", 128), ("b2", 128), ("b3", 128), ("b4", 3)):
+        p[OFF[b]:OFF[b] + n] = rng.normal(0, 0.1, n)
+    norm = np.concatenate([rng.normal(0, 0.3, 32), rng.uniform(0.5, 2.0, 32), rng.uniform(0.5, 3.0, 3)])
+    return p, norm
+
+
+def test_gradient_matches_central_differences():
+    p, norm = _net(1)
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (6, 32)).astype(np.float32)
+    T = rng.normal(0, 2, (6, 3)).astype(np.float32)
+    g, loss = O.mlp_grad(p, norm, X, T, bf16=False)
+    idx = np.concatenate([rng.choice(np.arange(OFF[a], OFF[b]), 6, replace=False)
+                          for a, b in (("W1", "b1"), ("b1", "W2"), ("W2", "b2"), ("b2", "W3"), ("W3", "b3"),
+                                       ("b3", "W4"), ("W4", "b4"))] + [np.arange(OFF["b4"], P)])
+    for k in idx:
+        e = 1e-6 * max(1.0, abs(p[k]))
+        q = p.copy()
+        q[k] += e
+        lp = O.mlp_grad(q, norm, X, T, bf16=False)[1]
+        q[k] -= 2 * e
+        lm = O.mlp_grad(q, norm, X, T, bf16=False)[1]
+        fd = (lp - lm) / (2 * e)
+        assert abs(fd - g[k]) <= 1e-6 * max(abs(g[k]), 1e-3), (k, fd, g[k])
+    # the loss is the mean squared normalised error of the forward pass
+    Y = O.mlp_forward(p, norm, X, bf16=False)
+    want = np.mean(((Y - T) / norm[64:67]) ** 2)
+    assert loss == pytest.approx(want, rel=1e-12)
+
+
+def test_bf16_emulation_stays_close_to_f64():
+    p, norm = _net(3)
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (64, 32)).astype(np.float32)
+    T = rng.normal(0, 2, (64, 3)).astype(np.float32)
+    g64, l64 = O.mlp_grad(p, norm, X, T, bf16=False)
+    g16, l16 = O.mlp_grad(p, norm, X, T, bf16=True)
+    assert np.linalg.norm(g16 - g64) <= 3e-2 * np.linalg.norm(g64)
+    assert l16 == pytest.approx(l64, rel=2e-2)
+    assert not np.array_equal(g16, g64)  # the emulation does round
+
+
+def test_adam_closed_form_two_steps():
+    rng = np.random.default_rng(5)
+    w0 = rng.normal(0, 1, 100)
+    g1, g2 = rng.normal(0, 1, 100), rng.normal(0, 1, 100)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    w, m, v = w0.copy(), np.zeros(100), np.zeros(100)
+    O.adam(w, m, v, g1, lr, b1, b2, eps, 1)
+    w1 = w0 - lr * g1 / (np.abs(g1) + eps)  # m_hat = g, v_hat = g^2 after one step
+    assert np.allclose(w, w1, rtol=0, atol=1e-15)
+    O.adam(w, m, v, g2, lr, b1, b2, eps, 2)
+    mh = (b1 * (1 - b1) * g1 + (1 - b1) * g2) / (1 - b1 ** 2)
+    vh = (b2 * (1 - b2) * g1 ** 2 + (1 - b2) * g2 ** 2) / (1 - b2 ** 2)
+    assert np.allclose(w, w1 - lr * mh / (np.sqrt(vh) + eps), rtol=0, atol=1e-15)
+
+
+def _feat(x, v, kind, rho=None, p=None, psi=None, rho0=1000.0, mass=0.125, h=0.1, c=20.0, B=5.7e4):
+    n = len(kind)
+    rho = np.full(n, rho0) if rho is None else rho
+    p = np.zeros(n) if p is None else p
+    psi = np.zeros(n) if psi is None else psi
+    return O.features(np.asarray(x, float), np.asarray(v, float), rho, p, np.asarray(kind), psi, rho0, mass, h, c, B)
+
+
+def test_features_closed_forms():
+    # a lone particle: density and pressure terms only
+    F, _ = _feat([[0, 0, 0]], [[1, 2, 3]], [0], rho=np.array([1010.0]), p=np.array([570.0]))
+    assert F[0, 0] == pytest.approx(0.01) and F[0, 1] == pytest.approx(0.01) and not F[0, 2:].any()
+    # one fluid partner at q = 0.9: shell floor(2.25) = 2, V = rho0 / rho_j
+    h, c = 0.1, 20.0
+    F, _ = _feat([[0, 0, 0], [0.09, 0, 0]], [[0, 0, 0], [0.4, 0, -0.2]], [0, 0], rho=np.array([1000.0, 1250.0]))
+    want = np.zeros(30)
+    want[12:18] = 0.8 * np.array([0.9, 0, 0, 0.4 / c, 0, -0.2 / c])
+    assert np.allclose(F[0, 2:], want, rtol=1e-14, atol=0)
+    # a boundary partner at q = 1.7 (shell 4) weighs psi rho0 / m
+    F, _ = _feat([[0, 0, 0], [0, 0.17, 0]], [[0, 0, 0], [0, 0, 0]], [0, 1], psi=np.array([0.0, 2e-4]))
+    assert F[0, 2 + 24 + 1] == pytest.approx(2e-4 * 1000.0 / 0.125 * 1.7, rel=1e-14)
+    assert not F[1].any()  # boundary rows are zero
+    # mirror-symmetric neighbourhood: no position moments; uniform velocity: no velocity moments
+    pts = [[0, 0, 0], [0.05, 0, 0], [-0.05, 0, 0], [0, 0.13, 0], [0, -0.13, 0], [0, 0, 0.07], [0, 0, -0.07]]
+    F, _ = _feat(pts, [[0.3, -0.1, 0.2]] * 7, [0] * 7)
+    assert np.allclose(F[0, 2:], 0.0, atol=1e-15)
+    # partners beyond 2h add nothing
+    F, _ = _feat([[0, 0, 0], [0.2 + 1e-9, 0, 0]], [[0, 0, 0], [1, 1, 1]], [0, 0])
+    assert not F[0, 2:].any()
+
+
+def test_features_flag_partners_on_shell_edges():
+    _, edge = _feat([[0, 0, 0], [0.04, 0, 0]], [[0, 0, 0]] * 2, [0, 0])  # q = 0.4: 2.5 q = 1
+    assert edge[0] and edge[1]
+    _, edge = _feat([[0, 0, 0], [0.05, 0, 0]], [[0, 0, 0]] * 2, [0, 0])
+    assert not edge.any()
