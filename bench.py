@@ -134,6 +134,12 @@ def main():
                     help="time every launch shape for this many steps first and keep the fastest")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this size (counter calibration)")
+    ap.add_argument("--surrogate", default=None,
+                    help="trained surrogate (.npz); default: the committed network of the configuration, if any")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--check-every", type=int, default=10, help="surrogate: plain-force check every k steps")
+    ap.add_argument("--accel-tol", type=float, default=0.2,
+                    help="surrogate: fall back when a check's acceleration error exceeds this")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is a design "
@@ -187,6 +193,7 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_fluid * args.steps / elapsed
     print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    rho_plain = W.densities() if parts is not None else None
 
     mean_entries, rebuilds0 = W.list_stats()
     diag = W.diagnostics()
@@ -247,9 +254,66 @@ def main():
         assert pef.lib().pefx_calibration_copy(args.calibration_copy_mib << 20) == 0
     if tune is not None:
         out["autotune"] = tune
+    sur_path = args.surrogate or os.path.join(ROOT, "physics-engine_amd", "pef", "models", f"surrogate_{cfg.name}.npz")
+    if parts is not None and not args.no_surrogate and os.path.exists(sur_path):
+        del W  # the plain world's memory is not needed any more
+        sur = surrogate_leg(cfg, opts, sur_path, rho_plain, kinds == 0, n_fluid, args, value)
+        out["surrogate"] = sur
+        out["config"]["surrogate_speedup"] = sur.get("speedup")
     if not args.no_cpu_baseline and parts is not None:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))
+
+
+def surrogate_leg(cfg, opts, path, rho_plain, fluid, n_fluid, args, value_plain):
+    """The same run with the learned force: warmup and timed steps on a fresh
+    world with the surrogate attached (plain-force check every k steps,
+    fallback on), its throughput, and its fluid densities after the run
+    against the plain FP32 run's (RMS relative error)."""
+    import numpy as np
+    import pef
+    from pef import scenes
+
+    m = pef.Mlp()
+    m.load(path)
+    parts, params = scenes.build(cfg)
+    W = pef.WcsphWorld(parts, params, opts)
+    W.set_surrogate(m, check_every=args.check_every, accel_tol=args.accel_tol, density_tol=0.005)
+    try:
+        W.step(cfg.dt, args.warmup)
+        t0 = time.perf_counter()
+        W.step(cfg.dt, args.steps)
+        elapsed = time.perf_counter() - t0
+    except pef.PefError as e:  # the predicted dynamics left the device grid
+        return {"diverged": str(e), "stats": W.surrogate_stats(), "network": os.path.relpath(path, ROOT)}
+    st = W.surrogate_stats()
+    rho = W.densities()
+    # per-kernel cost of a predicted step, from the state the run ended in:
+    # event times of the features pass, the inference and the update, beside
+    # the plain force of the check step
+    n_prof = 10
+    W.set_profiling(True)
+    W.set_surrogate(m, check_every=1 << 30, accel_tol=float("inf"), density_tol=float("inf"))
+    per = {}
+    try:
+        W.step(cfg.dt, 1 + n_prof)
+        times = W.kernel_times()
+        per = {k: round(v / n_prof, 5) for k, v in times.items() if k in ("features", "mlp", "integrate")}
+        per["force"] = round(times.get("force", 0.0), 5)
+    except pef.PefError:
+        pass
+    W.set_profiling(False)
+    err = float(np.sqrt(np.mean(((rho[fluid] - rho_plain[fluid]) / rho_plain[fluid]) ** 2)))
+    value = n_fluid * args.steps / elapsed
+    print(f"# surrogate: {1e3 * elapsed / args.steps:.3f} ms/step, density RMS vs plain {err:.2e}, {st}",
+          file=sys.stderr, flush=True)
+    return {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+            "speedup": round(value / value_plain, 4), "density_rms_vs_plain": err,
+            "meets_density_bound": bool(err <= 0.005), "check_every": args.check_every,
+            "accel_tol": args.accel_tol, "network": os.path.relpath(path, ROOT),
+            "surrogate_steps": st["surrogate_steps"], "plain_steps": st["plain_steps"], "checks": st["checks"],
+            "failed_checks": st["failed_checks"], "last_accel_error": round(st["last_accel_error"], 4),
+            "max_accel_error": round(st["max_accel_error"], 4), "predicted_step_kernel_ms": per}
 
 
 if __name__ == "__main__":

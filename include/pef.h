@@ -349,10 +349,9 @@ int32_t pef_world_sample(PefWorld* world, PefDataset* ds, uint64_t n, uint64_t s
 
 typedef struct PefSurrogateStats {
   uint64_t surrogate_steps; /* steps integrated with the predicted acceleration */
-  uint64_t plain_steps;     /* plain steps while attached: checks and steps after a fallback */
+  uint64_t plain_steps;     /* plain steps while attached: checks and fallback steps */
   uint64_t checks;
-  int32_t fallback;         /* 1 once a check failed; every later step is plain */
-  int32_t pad;
+  uint64_t failed_checks;   /* checks that sent the steps up to the next check to the plain force */
   double last_accel_error;  /* RMS |a_pred - (a - g)| / RMS |a - g| over fluid, last check */
   double max_accel_error;
   double last_density_rms;  /* RMS (rho / rho0 - 1) over fluid at the last check */
@@ -360,11 +359,12 @@ typedef struct PefSurrogateStats {
 } PefSurrogateStats;
 /* Attach (mlp non-NULL) or detach the surrogate of a single-scene world. Step k
  * after attaching is a check step when k % check_every == 0: the plain force
- * integrates it and the prediction for the same state is compared. A check with
- * an acceleration error above accel_tol, or a density RMS above the first
- * check's by more than density_tol, latches the fallback. Other steps integrate
- * the predicted acceleration (boundary particles then get no reaction force).
- * The world keeps the pointer: keep the network alive while attached. */
+ * integrates it and the prediction for the same state is compared. A check
+ * fails when the acceleration error exceeds accel_tol or the density RMS
+ * exceeds the first check's by more than density_tol; the steps up to the next
+ * check then fall back to the plain force, and a passing check hands them back
+ * to the prediction (boundary particles get no reaction force on predicted
+ * steps). The world keeps the pointer: keep the network alive while attached. */
 int32_t pef_world_set_surrogate(PefWorld* world, PefMlp* mlp, uint32_t check_every, double accel_tol,
                                 double density_tol);
 int32_t pef_world_surrogate_stats(PefWorld* world, PefSurrogateStats* out);

@@ -213,9 +213,11 @@ __device__ inline void stage_halo(const HaloTab& T, uint32_t H, float4* __restri
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       const uint32_t t = t0 + u * blockDim.x;
-      const uint32_t s = halo_slot(T, t < H ? t : H - 1u);
-      a[u] = ga[s];
-      if (NARR == 2) b[u] = gb[s];
+      if (t < H) {
+        const uint32_t s = halo_slot(T, t);
+        a[u] = ga[s];
+        if (NARR == 2) b[u] = gb[s];
+      }
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {

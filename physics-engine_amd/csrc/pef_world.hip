@@ -95,6 +95,7 @@ class World {
   uint32_t sur_every = 10;
   double sur_accel_tol = 0.25, sur_density_tol = 0.005;
   uint64_t sur_k = 0;         // steps since attaching
+  bool sur_fallback = false;  // the last check failed: plain force up to the next check
   PefSurrogateStats sur_stats{};
 
   ~World() { release(); }
@@ -637,8 +638,9 @@ void fresh_pressure(World& w) {
 
 // Steps with the surrogate attached, launched eagerly: a check step runs the
 // features, the prediction and the plain force (which integrates) and compares
-// them; other steps integrate the prediction. The rest of the body (rebuild,
-// density, XSPH) is the plain step's.
+// them; the steps up to the next check integrate the prediction, or the plain
+// force when the check failed. The rest of the body (rebuild, density, XSPH)
+// is the plain step's.
 int32_t do_step_surrogate(World& w, double dt, uint32_t nsteps) {
   int32_t rc = sur_buffers(w);
   if (rc) return rc;
@@ -646,26 +648,40 @@ int32_t do_step_surrogate(World& w, double dt, uint32_t nsteps) {
   std::vector<double> part((size_t)nb * 4);
   PefSurrogateStats& st = w.sur_stats;
   for (uint32_t k = 0; k < nsteps; ++k) {
-    const bool fallback = st.fallback != 0;
-    const bool check = !fallback && (w.sur_k % w.sur_every == 0);
-    if (w.any_adapt) pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+    const bool check = w.sur_k % w.sur_every == 0;
+    const bool fallback = !check && w.sur_fallback;
+    Prof pr(w);
+    pr.mark("start");
+    if (w.any_adapt) {
+      pef::launch_pressure(w.stream, w.N, w.sc_d, w.cur, w.st_d, w.S, nullptr);
+      pr.mark("pressure");
+    }
     if (fallback) {
       launch_force(w, 1);
+      pr.mark("force");
     } else {
       launch_features(w);
+      pr.mark("features");
       pef::launch_mlp_infer(w.stream, w.sur->m.dev(), w.FEAT, pef::kSurFeatures, w.N, w.PRED, w.cur.META,
                             pef::kMetaBoundary, w.tc.n_cu);
+      pr.mark("mlp");
       if (check) {
         launch_force(w, 1);
+        pr.mark("force");
         pef::launch_sur_check(w.stream, w.N, w.sc_d, w.cur, w.PRED, w.sur_part);
+        pr.mark("check");
       } else {
         pef::launch_integrate_pred(w.stream, w.N, w.ctl_d, w.ctl_d, w.sc_d, w.cur, w.PRED);
+        pr.mark("integrate");
       }
     }
-    enqueue_rebuild(w, nullptr);
+    enqueue_rebuild(w, &pr);
     launch_density(w);
+    pr.mark("density");
     launch_xsph(w);
+    pr.mark("xsph");
     HIPTRY(hipGetLastError());
+    pr.finish();
     w.t += dt;
     ++w.sur_k;
     if (fallback) {
@@ -688,9 +704,9 @@ int32_t do_step_surrogate(World& w, double dt, uint32_t nsteps) {
       st.max_accel_error = std::max(st.max_accel_error, st.last_accel_error);
       st.last_density_rms = n > 0.0 ? std::sqrt(d2 / n) : 0.0;
       if (st.checks == 1) st.ref_density_rms = st.last_density_rms;
-      if (!(st.last_accel_error <= w.sur_accel_tol) ||
-          !(st.last_density_rms <= st.ref_density_rms + w.sur_density_tol))
-        st.fallback = 1;
+      w.sur_fallback = !(st.last_accel_error <= w.sur_accel_tol) ||
+                       !(st.last_density_rms <= st.ref_density_rms + w.sur_density_tol);
+      if (w.sur_fallback) ++st.failed_checks;
     }
   }
   return PEF_OK;
@@ -1149,6 +1165,7 @@ int32_t set_surrogate(World& w, PefMlp* mlp, uint32_t check_every, double accel_
   w.sur_accel_tol = accel_tol;
   w.sur_density_tol = density_tol;
   w.sur_k = 0;
+  w.sur_fallback = false;
   w.sur_stats = PefSurrogateStats{};
   return PEF_OK;
 }

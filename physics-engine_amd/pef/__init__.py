@@ -117,7 +117,7 @@ _lib = None
 
 class PefSurrogateStats(C.Structure):
     _fields_ = [("surrogate_steps", C.c_uint64), ("plain_steps", C.c_uint64), ("checks", C.c_uint64),
-                ("fallback", C.c_int32), ("pad", C.c_int32), ("last_accel_error", C.c_double),
+                ("failed_checks", C.c_uint64), ("last_accel_error", C.c_double),
                 ("max_accel_error", C.c_double), ("last_density_rms", C.c_double),
                 ("ref_density_rms", C.c_double)]
 
@@ -602,7 +602,7 @@ class WcsphWorld:
     def surrogate_stats(self) -> dict:
         st = PefSurrogateStats()
         _check(lib().pef_world_surrogate_stats(self._h, C.byref(st)), "surrogate_stats")
-        return {k: getattr(st, k) for k, _ in PefSurrogateStats._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in PefSurrogateStats._fields_}
 
     def tile_info(self) -> dict:
         """Launch configuration of the brick-tiled kernels and brick counts."""
