@@ -179,6 +179,9 @@ def main():
     else:
         workload = (f"{cfg.name}: {n_fluid} fluid + {n_bnd} Akinci wall particles, dx {cfg.dx}, h {cfg.h}, "
                     f"dt {cfg.dt}, tank {cfg.tank}, jitter {cfg.jitter} dx seed {cfg.seed}")
+        if cfg.obstacle is not None:
+            workload += (f", static box obstacle centre {cfg.obstacle[0]} half extents {cfg.obstacle[1]} "
+                         f"(one-sided samples, psi = dx^3)")
     if args.steps is None:
         args.steps = cfg.steps
     if args.autotune:
@@ -254,13 +257,17 @@ def main():
         assert pef.lib().pefx_calibration_copy(args.calibration_copy_mib << 20) == 0
     if tune is not None:
         out["autotune"] = tune
-    sur_path = args.surrogate or os.path.join(ROOT, "physics-engine_amd", "pef", "models", f"surrogate_{cfg.name}.npz")
-    if parts is not None and not args.no_surrogate and os.path.exists(sur_path):
+    # the network trained on the 1M dam break serves every 3-D dam break at its spacing
+    sur_path = args.surrogate or os.path.join(ROOT, "physics-engine_amd", "pef", "models",
+                                              "surrogate_dam_break_1m.npz")
+    if args.surrogate is None and (cfg.dim != 3 or cfg.dx != scenes.GPU_1M.dx):
+        sur_path = ""
+    if parts is not None and not args.no_surrogate and sur_path and os.path.exists(sur_path):
         del W  # the plain world's memory is not needed any more
         sur = surrogate_leg(cfg, opts, sur_path, rho_plain, kinds == 0, n_fluid, args, value)
         out["surrogate"] = sur
         out["config"]["surrogate_speedup"] = sur.get("speedup")
-    if not args.no_cpu_baseline and parts is not None:
+    if not args.no_cpu_baseline and cfg.name == scenes.GPU_1M.name:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))
 

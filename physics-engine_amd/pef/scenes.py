@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, replace
-from typing import List, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -92,6 +92,8 @@ class DamBreak:
     dim: int = 3
     wall_gap: float = 2.0  # distance of the first fluid layer from the walls, in dx
     adapt_sound_speed: bool = False
+    # static box obstacle (centre, half extents) sampled like FluidRigidCoupler::sample_box
+    obstacle: Optional[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = None
 
     @property
     def h(self) -> float:
@@ -112,7 +114,12 @@ GPU_1M = DamBreak("dam_break_1m", 128, 64, 128, 0.005, (1.6, 0.335, 0.8), 2.5e-5
 # 8,388,608-particle large run: 256 x 128 x 256 at dx 0.005.
 GPU_8M = DamBreak("dam_break_8m", 256, 128, 256, 0.005, (3.2, 0.655, 1.6), 2.5e-5, 2000)
 
-CONFIGS = {c.name: c for c in (CPU_8K, CHECK_64K, GPU_1M, GPU_8M)}
+# The large hybrid run: the 8M dam break with a static 0.2 x 0.2 x 0.3 m box resting on the floor at
+# the tank's x-y centre (12 802 one-sided samples).
+HYBRID_8M = DamBreak("dam_break_8m_obstacle", 256, 128, 256, 0.005, (3.2, 0.655, 1.6), 2.5e-5, 2000,
+                     obstacle=((1.6, 0.3275, 0.15), (0.1, 0.1, 0.15)))
+
+CONFIGS = {c.name: c for c in (CPU_8K, CHECK_64K, GPU_1M, GPU_8M, HYBRID_8M)}
 
 
 def _round_half_away(x: float) -> int:
@@ -173,7 +180,66 @@ def build(cfg: DamBreak):
         pef.jitter_positions(parts, cfg.seed, cfg.jitter * cfg.dx)
     pef.fill_box_boundary(parts, (0.0, 0.0, 0.0), cfg.tank, cfg.dx, 1)
     pef.recomputed_akinci_psi(parts)
+    if cfg.obstacle is not None:  # rigid samples keep psi = dx^dim, as the coupler samples them
+        sample_box(parts, cfg.obstacle[0], cfg.obstacle[1], cfg.dx)
     params = pef.WcsphParams.water(cfg.rest_density, cfg.sound_speed, cfg.gravity)
     params.visc_alpha = cfg.visc_alpha
     params.adapt_sound_speed = cfg.adapt_sound_speed
     return parts, params
+
+
+def sample_box(parts, centre, half, dx: float, layers: int = 1):
+    """FluidRigidCoupler::sample_box (coupling.rs:103-231) for a body at rest at
+    `centre`: one-sided Akinci samples of an axis-aligned box surface with
+    outward face normals and psi = dx^dim (RIGID_PSI_SCALE = 1), appended in
+    the reference's loop order (z faces, then y faces, then x faces per layer;
+    2-D: y faces, then x faces)."""
+    import pef
+
+    cx, cy, cz = (float(v) for v in centre)
+    hx, hy, hz = (float(v) for v in half)
+    psi = dx * dx if parts.dim == pef.D2 else dx * dx * dx
+    layers = max(layers, 1)
+    xs, ns = [], []
+
+    def push(p, n):
+        xs.append((cx + p[0], cy + p[1], cz + p[2]))
+        ns.append(n)
+
+    nx = max(_round_half_away(2.0 * hx / dx), 1)
+    ny = max(_round_half_away(2.0 * hy / dx), 1)
+    for layer in range(layers):
+        off = layer * dx
+        if parts.dim == pef.D2:
+            for i in range(nx + 1):
+                x = -hx + i * (2.0 * hx) / nx
+                push((x, -hy - off, 0.0), (0.0, 0.0, 0.0))
+                push((x, hy + off, 0.0), (0.0, 0.0, 0.0))
+            for j in range(1, ny):
+                y = -hy + j * (2.0 * hy) / ny
+                push((-hx - off, y, 0.0), (0.0, 0.0, 0.0))
+                push((hx + off, y, 0.0), (0.0, 0.0, 0.0))
+            continue
+        nz = max(_round_half_away(2.0 * hz / dx), 1)
+        for i in range(nx + 1):
+            for j in range(ny + 1):
+                x = -hx + i * (2.0 * hx) / nx
+                y = -hy + j * (2.0 * hy) / ny
+                push((x, y, -hz - off), (0.0, 0.0, -1.0))
+                push((x, y, hz + off), (0.0, 0.0, 1.0))
+        for i in range(nx + 1):
+            for k in range(1, nz):
+                x = -hx + i * (2.0 * hx) / nx
+                z = -hz + k * (2.0 * hz) / nz
+                push((x, -hy - off, z), (0.0, -1.0, 0.0))
+                push((x, hy + off, z), (0.0, 1.0, 0.0))
+        for j in range(1, ny):
+            for k in range(1, nz):
+                y = -hy + j * (2.0 * hy) / ny
+                z = -hz + k * (2.0 * hz) / nz
+                push((-hx - off, y, z), (-1.0, 0.0, 0.0))
+                push((hx + off, y, z), (1.0, 0.0, 0.0))
+    n = len(xs)
+    parts._append(np.asarray(xs, float), np.zeros((n, 3)), np.full(n, pef.BOUNDARY, np.uint8), np.full(n, psi),
+                  np.asarray(ns, float))
+    return n

@@ -84,3 +84,25 @@ def test_seed_dam_break_particles_layout():
     # small reservoirs are clamped to six particles a side
     p2, d2 = V.seed_dam_break_particles(0.05, 0.1, 0.1, 0.5)
     assert p2.fluid_count() == 36 and d2 == 6 * 0.05
+
+
+def test_sample_box_matches_the_reference_layout():
+    # coupling.rs:103-231: per layer 2 (nx+1)(ny+1) + 2 (nx+1)(nz-1) + 2 (ny-1)(nz-1) samples;
+    # the survey's 0.2 x 0.2 x 0.3 m box at dx 5 mm has 12 802
+    from pef import scenes
+    P = pef.Particles(pef.D3, 1000.0, 0.125, 0.01)
+    n = scenes.sample_box(P, (1.6, 0.3275, 0.15), (0.1, 0.1, 0.15), 0.005)
+    assert n == len(P) == 12802
+    x, nrm = P.x, P.n_out
+    assert np.allclose(x.min(0), [1.5, 0.2275, 0.0]) and np.allclose(x.max(0), [1.7, 0.4275, 0.3])
+    assert np.allclose(np.linalg.norm(nrm, axis=1), 1.0)
+    # every sample lies on the face its normal points out of
+    c, h = np.array([1.6, 0.3275, 0.15]), np.array([0.1, 0.1, 0.15])
+    a = np.abs(nrm).argmax(1)
+    face = c[a] + np.sign(nrm[np.arange(n), a]) * h[a]
+    assert np.allclose(x[np.arange(n), a], face)
+    assert np.all(P.psi == 0.005 ** 3) and np.all(P.kind == pef.BOUNDARY)
+    # first samples: the z faces of i = 0, j = 0, bottom then top
+    assert np.allclose(x[:2], [[1.5, 0.2275, 0.0], [1.5, 0.2275, 0.3]])
+    Q = pef.Particles(pef.D2, 1000.0, 0.025, 0.01)
+    assert scenes.sample_box(Q, (0.5, 0.5, 0.0), (0.1, 0.05, 0.0), 0.01) == 2 * 21 + 2 * 9
