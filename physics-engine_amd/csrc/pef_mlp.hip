@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -153,6 +154,17 @@ __device__ inline bf16x8 input_frag(const float* F, uint64_t row, uint32_t fstri
   return frag_pack(x);
 }
 
+// Raise a kernel's dynamic-LDS limit once per device (the attribute is per
+// device); thread-safe through an atomic bit per device.
+static void lds_limit_once(std::atomic<uint64_t>& done, const void* fn, size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  const uint64_t bit = 1ull << dev;
+  if (done.load(std::memory_order_acquire) & bit) return;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    done.fetch_or(bit, std::memory_order_acq_rel);
+}
+
 constexpr int kInferThreads = 1024;
 constexpr size_t kInferLds = kPackedElems * 2 + kBiasElems * 4 + (kInferThreads / 64) * 16 * kLdH * 2;
 
@@ -209,12 +221,8 @@ __global__ void __launch_bounds__(kInferThreads)
 void launch_mlp_infer(hipStream_t st, const MlpDev& m, const float* features, uint32_t fstride, uint32_t n,
                       float4* out, const uint32_t* meta, uint32_t skip_mask, uint32_t n_cu) {
   if (n == 0) return;
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kInferLds);
-    attr = true;
-  }
+  static std::atomic<uint64_t> attr{0};
+  lds_limit_once(attr, (const void*)k_mlp_infer, kInferLds);
   const uint32_t tiles = (n + 15) / 16, per = kInferThreads / 64;
   const uint32_t grid = std::max(1u, std::min(n_cu, (tiles + per - 1) / per));
   hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferThreads), kInferLds, st, m, features, fstride, n, out,
@@ -405,12 +413,8 @@ __global__ void __launch_bounds__(kTrainThreads)
 
 void launch_mlp_train_fb(hipStream_t st, const MlpDev& m, const float* feat, const float* targ,
                          const uint32_t* idx, const MlpTrainBufs& tb, uint32_t n_cu) {
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_mlp_train_fb, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kTrainLds);
-    attr = true;
-  }
+  static std::atomic<uint64_t> attr{0};
+  lds_limit_once(attr, (const void*)k_mlp_train_fb, kTrainLds);
   const uint32_t tiles = tb.B / 16, per = kTrainThreads / 64;
   const uint32_t grid = std::max(1u, std::min(n_cu, (tiles + per - 1) / per));
   hipLaunchKernelGGL(k_mlp_train_fb, dim3(grid), dim3(kTrainThreads), kTrainLds, st, m, feat, targ, idx, tb);
