@@ -113,6 +113,7 @@ def lib():
         "ora_world_diagnostics": (None, [vp, d, C.POINTER(OraDiag)]),
         "ora_fnv1a_f64": (u64, [vp, u64]),
         "ora_fnv1a_bytes": (u64, [vp, u64]),
+        "ora_dfsph_tank_hash": (u64, [u64]),
         "ora_mlp_forward": (None, [vp, vp, vp, u64, i32, vp]),
         "ora_mlp_grad": (None, [vp, vp, vp, vp, u64, i32, vp, pd]),
         "ora_adam": (None, [vp, vp, vp, vp, u64, d, d, d, d, C.c_uint32]),
@@ -172,6 +173,11 @@ def mass_from_spacing(rho0, dx, dim=3):
 
 def mass_calibrated_to_rest(rho0, dx, h, dim=3):
     return lib().ora_mass_calibrated_to_rest(rho0, dx, h, dim)
+
+
+def dfsph_tank_hash(steps=40):
+    """particles_hash64 of the canonical DFSPH tank after `steps` steps (scenarios.rs:72-125)."""
+    return lib().ora_dfsph_tank_hash(steps)
 
 
 def fnv1a_f64(vals):

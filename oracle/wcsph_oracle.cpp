@@ -28,6 +28,11 @@
 //                                    The bitwise f64 trajectory is parity unpinned
 //                                    (no reference hash or golden vector exists).
 //   FNV-1a hash .................... published FNV-1a 64 test vectors.
+//   shared pieces, bit for bit ..... the reference's locked DFSPH tank hash
+//                                    0xf1f2825b5eeaab0d (docs/DETERMINISM.md:
+//                                    644-661), reproduced by the DFSPH
+//                                    restatement below on the same grid,
+//                                    kernel, psi, fill and hash functions.
 //   xoshiro256** / SplitMix64 ...... published reference outputs.
 //   fill_box / fill_box_boundary ... closed-form particle counts.
 //   recomputed_akinci_psi .......... O(B^2) strict vs grid form bit identity.
@@ -804,6 +809,289 @@ struct Rng {
   double uniform(double lo, double hi) { return lo + (hi - lo) * next_f64(); }
 };
 
+// ---------------------------------------------------------------------------
+// DFSPH world (crates/pe-fluid/src/dfsph.rs), fixed-dt step (dfsph.rs:704-760)
+// with the same grid (cell = support = 2h), the same unfiltered candidate
+// lists in the reference's order, the cubic spline, ψ-weighted Akinci
+// boundary masses and one-sided filter. Boundary reaction forces are not
+// restated (they never feed back into the fluid); surface tension, the
+// velocity cap and free-slip planes are the off defaults of the restated
+// scenario and are not restated either.
+// ---------------------------------------------------------------------------
+struct DfsphParams {
+  int max_density_iters = 50, max_divergence_iters = 50;  // dfsph.rs:78-98 defaults
+  double density_error_tol = 0.001, divergence_error_tol = 0.001;
+  V3 gravity{0.0, 0.0, -9.80665};
+  double xsph_eps = 0.05;
+  double particle_diameter = 0.0;
+  bool warm_start = true;
+  double solver_omega = 1.0;
+};
+
+struct Dfsph {
+  Particles P;
+  Grid grid;
+  DfsphParams prm;
+  double t = 0.0, last_dt = 0.0;
+  std::vector<double> alpha, kappa, kappa_v, kappa_warm, kappa_v_warm;
+  std::vector<V3> v_star, a_adv, scratch_dv;
+  std::vector<std::vector<uint64_t>> nl;
+
+  void init() {  // dfsph.rs:140-180
+    if (prm.particle_diameter <= 0.0) prm.particle_diameter = P.h;
+    grid.cell_size = 2.0 * P.h;  // support_radius(CubicSpline, h), kernels.rs:32-37
+    grid.dim = P.dim;
+    const size_t n = P.len();
+    if (P.dim == DIM2)
+      for (size_t i = 0; i < n; ++i) {
+        P.x[i].z = 0.0;
+        P.v[i].z = 0.0;
+      }
+    alpha.assign(n, 0.0);
+    kappa.assign(n, 0.0);
+    kappa_v.assign(n, 0.0);
+    kappa_warm.assign(n, 0.0);
+    kappa_v_warm.assign(n, 0.0);
+    v_star.assign(n, v3(0, 0, 0));
+    a_adv.assign(n, v3(0, 0, 0));
+    scratch_dv.assign(n, v3(0, 0, 0));
+    nl.assign(n, {});
+    rebuild_neighbours();
+    compute_density();
+    compute_alpha();
+  }
+  bool fluid(size_t i) const { return P.kind[i] == KIND_FLUID; }
+
+  void rebuild_neighbours() {  // dfsph.rs:199-209
+    grid.rebuild(P.x);
+    for (size_t i = 0; i < P.len(); ++i) {
+      nl[i].clear();
+      if (fluid(i)) grid.neighbours(i, nl[i]);
+    }
+  }
+
+  void compute_density() {  // dfsph.rs:212-239
+    const double h = P.h, mass = P.mass, rho0 = P.rest_density;
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) {
+        P.rho[i] = rho0;
+        continue;
+      }
+      const V3 xi = P.x[i];
+      double rho = mass * w(0.0, h, P.dim);
+      for (uint64_t j : nl[i]) {
+        if (j == i) continue;
+        const double r = norm(sub(P.x[j], xi));
+        const double wij = w(r, h, P.dim);
+        if (fluid(j)) rho += mass * wij;
+        else if (P.boundary_sees_fluid(j, xi)) rho += rho0 * P.psi[j] * wij;
+      }
+      P.rho[i] = std::fmax(rho, 1e-12);
+    }
+  }
+
+  void compute_alpha() {  // dfsph.rs:247-299
+    const double h = P.h, mass = P.mass, rho0 = P.rest_density;
+    const double hs = std::fmax(h, 1e-12);
+    const double char_grad = (P.dim == DIM2) ? mass / (hs * hs * hs) : mass / (hs * hs * hs * hs);
+    const double floor_ = std::fmax(1e-6 * char_grad * char_grad, 1e-12);
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) {
+        alpha[i] = 0.0;
+        continue;
+      }
+      const V3 xi = P.x[i];
+      V3 sg = v3(0, 0, 0);
+      double sq = 0.0;
+      size_t nn = 0;
+      for (uint64_t j : nl[i]) {
+        if (j == i) continue;
+        const V3 gw = grad_w(sub(P.x[j], xi), h, P.dim);
+        double mj;
+        if (fluid(j)) mj = mass;
+        else mj = P.boundary_sees_fluid(j, xi) ? rho0 * P.psi[j] : 0.0;
+        if (mj == 0.0 && !fluid(j)) continue;
+        const V3 g = mul(gw, mj);
+        sg = add(sg, g);
+        sq += norm2(g);
+        nn += 1;
+      }
+      if (nn < 2) {
+        alpha[i] = 0.0;
+        continue;
+      }
+      const double den = std::fmax(norm2(sg) + sq, floor_);
+      alpha[i] = P.rho[i] / den;
+    }
+  }
+
+  double neighbour_mass(size_t i, size_t j) const {  // dfsph.rs:301-315
+    if (fluid(j)) return P.mass;
+    return P.boundary_sees_fluid(j, P.x[i]) ? P.rest_density * P.psi[j] : 0.0;
+  }
+
+  double density_derivative(size_t i) const {  // dfsph.rs:318-353
+    const V3 xi = P.x[i], vi = v_star[i];
+    double drho = 0.0;
+    for (uint64_t j : nl[i]) {
+      if (j == i) continue;
+      const double mj = neighbour_mass(i, j);
+      if (mj == 0.0) continue;
+      const V3 gw = grad_w(sub(P.x[j], xi), P.h, P.dim);
+      const V3 vj = fluid(j) ? v_star[j] : P.v[j];
+      drho += mj * dot(sub(vi, vj), gw);
+    }
+    return drho;
+  }
+
+  void apply_xsph_on_vstar() {  // dfsph.rs:381-417
+    const double eps = prm.xsph_eps;
+    if (eps == 0.0) return;
+    const double h = P.h, mass = P.mass;
+    for (auto& d : scratch_dv) d = v3(0, 0, 0);
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) continue;
+      const V3 xi = P.x[i], vi = v_star[i];
+      const double rho_i = P.rho[i];
+      V3 dv = v3(0, 0, 0);
+      for (uint64_t j : nl[i]) {
+        if (j == i || !fluid(j)) continue;
+        const double r = norm(sub(P.x[j], xi));
+        const double wij = w(r, h, P.dim);
+        const double rho_bar = 0.5 * (rho_i + P.rho[j]);
+        dv = add(dv, mul(sub(v_star[j], vi), mass / std::fmax(rho_bar, 1e-12) * wij));
+      }
+      scratch_dv[i] = mul(dv, eps);
+    }
+    for (size_t i = 0; i < P.len(); ++i)
+      if (fluid(i)) v_star[i] = add(v_star[i], scratch_dv[i]);
+  }
+
+  // v* -= dt sum_j m_j (k_i / rho_i + k_j / rho_j) grad W (dfsph.rs:423-516),
+  // Jacobi: every particle's correction from the same v*.
+  void apply_pressure_velocity_update(double dt, bool use_div) {
+    const std::vector<double>& K = use_div ? kappa_v : kappa;
+    const double h = P.h;
+    for (auto& d : scratch_dv) d = v3(0, 0, 0);
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) continue;
+      const V3 xi = P.x[i];
+      const double rho_i = std::fmax(P.rho[i], 1e-12), ki = K[i];
+      V3 dv = v3(0, 0, 0);
+      for (uint64_t j : nl[i]) {
+        if (j == i) continue;
+        const double mj = neighbour_mass(i, j);
+        if (mj == 0.0) continue;
+        const V3 gw = grad_w(sub(P.x[j], xi), h, P.dim);
+        const double kj = fluid(j) ? K[j] : 0.0;
+        const double rho_j = fluid(j) ? std::fmax(P.rho[j], 1e-12) : P.rest_density;
+        const double coeff = mj * (ki / rho_i + kj / rho_j);
+        dv = add(dv, mul(gw, coeff));
+      }
+      scratch_dv[i] = mul(dv, -dt);
+    }
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) continue;
+      v_star[i] = add(v_star[i], scratch_dv[i]);
+      if (P.dim == DIM2) v_star[i].z = 0.0;
+    }
+  }
+
+  int correct_density_error(double dt) {  // dfsph.rs:519-578
+    const double rho0 = P.rest_density, dt2 = dt * dt;
+    int iters = 0;
+    if (prm.warm_start) {
+      kappa = kappa_warm;
+      apply_pressure_velocity_update(dt, false);
+    }
+    std::fill(kappa_warm.begin(), kappa_warm.end(), 0.0);
+    for (;;) {
+      double avg = 0.0;
+      size_t count = 0;
+      for (size_t i = 0; i < P.len(); ++i) {
+        if (!fluid(i)) {
+          kappa[i] = 0.0;
+          continue;
+        }
+        const double rho_star = P.rho[i] + dt * density_derivative(i);
+        const double err = std::fmax(rho_star - rho0, 0.0);
+        avg += err / rho0;
+        count += 1;
+        kappa[i] = prm.solver_omega * err / dt2 * alpha[i];
+        if (prm.warm_start) kappa_warm[i] += kappa[i];
+      }
+      if (count > 0) avg /= (double)count;
+      apply_pressure_velocity_update(dt, false);
+      iters += 1;
+      if ((avg <= prm.density_error_tol && iters >= 2) || iters >= prm.max_density_iters) break;
+    }
+    return iters;
+  }
+
+  int correct_divergence_error(double dt) {  // dfsph.rs:581-621
+    const double rho0 = P.rest_density;
+    int iters = 0;
+    if (prm.warm_start) {
+      kappa_v = kappa_v_warm;
+      apply_pressure_velocity_update(dt, true);
+    }
+    std::fill(kappa_v_warm.begin(), kappa_v_warm.end(), 0.0);
+    for (;;) {
+      double avg = 0.0;
+      size_t count = 0;
+      for (size_t i = 0; i < P.len(); ++i) {
+        if (!fluid(i)) {
+          kappa_v[i] = 0.0;
+          continue;
+        }
+        const double drho = std::fmax(density_derivative(i), 0.0);
+        avg += drho / rho0;
+        count += 1;
+        kappa_v[i] = prm.solver_omega * (1.0 / dt) * drho * alpha[i];
+        if (prm.warm_start) kappa_v_warm[i] += kappa_v[i];
+      }
+      if (count > 0) avg /= (double)count;
+      apply_pressure_velocity_update(dt, true);
+      iters += 1;
+      if ((avg <= prm.divergence_error_tol && iters >= 1) || iters >= prm.max_divergence_iters) break;
+    }
+    return iters;
+  }
+
+  void step(double dt) {  // dfsph.rs:704-760
+    dt = std::fmax(dt, 1e-12);
+    last_dt = dt;
+    const size_t n = P.len();
+    for (size_t i = 0; i < n; ++i) a_adv[i] = fluid(i) ? prm.gravity : v3(0, 0, 0);  // dfsph.rs:355-379
+    for (size_t i = 0; i < n; ++i) {
+      if (!fluid(i)) {
+        v_star[i] = P.v[i];
+        continue;
+      }
+      v_star[i] = add(P.v[i], mul(a_adv[i], dt));
+      if (P.dim == DIM2) v_star[i].z = 0.0;
+    }
+    apply_xsph_on_vstar();
+    correct_density_error(dt);
+    for (size_t i = 0; i < n; ++i) {
+      if (!fluid(i)) continue;
+      P.x[i] = add(P.x[i], mul(v_star[i], dt));
+      if (P.dim == DIM2) P.x[i].z = 0.0;
+    }
+    rebuild_neighbours();
+    compute_density();
+    compute_alpha();
+    correct_divergence_error(dt);
+    for (size_t i = 0; i < n; ++i) {
+      if (!fluid(i)) continue;
+      P.v[i] = v_star[i];
+      P.p[i] = kappa[i];
+      if (P.dim == DIM2) P.v[i].z = 0.0;
+    }
+    t += dt;
+  }
+};
+
 // FNV-1a over little-endian f64 bits of fluid (x, v) (diagnostics.rs:7-33).
 static uint64_t particles_hash64(const Particles& P) {
   uint64_t hsh = 0xcbf29ce484222325ULL;
@@ -835,6 +1123,35 @@ static uint64_t particles_hash64(const Particles& P) {
 using namespace ora;
 
 extern "C" {
+
+// The canonical DFSPH tank (scenarios.rs:72-117): 4x4x4 fluid lattice at
+// dx = 0.05, h = 2dx, rho0 = 1000, mass calibrated to the cubic lattice, in a
+// one-layer Akinci shell 0.4 x 0.4 x 0.45 with recomputed psi; `steps` steps
+// of dt = 1/120; returns particles_hash64 (the reference locks
+// 0xf1f2825b5eeaab0d for 40 steps).
+uint64_t ora_dfsph_tank_hash(uint64_t steps) {
+  using namespace ora;
+  const double dx = 0.05, h = 2.0 * dx, rho0 = 1000.0;
+  Dfsph D;
+  D.P.dim = DIM3;
+  D.P.rest_density = rho0;
+  D.P.mass = mass_calibrated_to_rest(rho0, dx, h, DIM3);
+  D.P.h = h;
+  fill_box(D.P, v3(0.1, 0.1, 0.1), 4, 4, 4, dx, v3(0, 0, 0));
+  fill_box_boundary(D.P, v3(0.0, 0.0, 0.0), v3(0.4, 0.4, 0.45), dx, 1);
+  recomputed_akinci_psi(D.P, true);
+  D.prm.gravity = v3(0.0, 0.0, -9.80665);
+  D.prm.particle_diameter = dx;
+  D.prm.density_error_tol = 0.01;
+  D.prm.divergence_error_tol = 0.01;
+  D.prm.max_density_iters = 8;
+  D.prm.max_divergence_iters = 8;
+  D.prm.xsph_eps = 0.05;
+  D.prm.warm_start = true;
+  D.init();
+  for (uint64_t k = 0; k < steps; ++k) D.step(1.0 / 120.0);
+  return particles_hash64(D.P);
+}
 
 typedef struct {
   int32_t dim;
