@@ -238,6 +238,29 @@ int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEn
  * and number of list rebuilds since creation. */
 int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds);
 
+/* ---- rigid-body samples: the coupler's side of the step (coupling.rs:267-331) */
+
+/* Register the FluidRigidCoupler samples (coupling.rs:267-286): sample k is
+ * boundary particle particle[k] (original index) of body body[k] < n_bodies,
+ * at body-frame offset local[3k..3k+2] with body-frame normal
+ * local_n[3k..3k+2]. Replaces any previous registration. */
+int32_t pef_world_set_rigid_samples(PefWorld* world, uint64_t n, const uint64_t* particle, const uint32_t* body,
+                                    const double* local, const double* local_n, uint32_t n_bodies);
+/* FluidRigidCoupler::sync_from_rigid (coupling.rs:289-305) on the device:
+ * body b at position x[3b..] and unit quaternion q[4b..] = (w, x, y, z) puts
+ * its samples at x + R(q) local with normals R(q) local_n (FP64, the
+ * reference's operation order; 2-D zeroes z). Then, as the reference's next
+ * step begins, lists (when the skin test asks), densities and pressures are
+ * brought up to date. Sample velocities are not taken: the WCSPH step never
+ * reads a boundary particle's velocity (wcsph.rs:117-323). */
+int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q);
+/* FluidRigidCoupler::reduce_with_state (coupling.rs:308-331) on the device:
+ * force[3b..] = sum of the samples' boundary_force and torque[3b..] = sum of
+ * (x_s - com_b) x f_s, FP64 with a fixed summation tree, from the last force
+ * evaluation. */
+int32_t pef_world_reduce_rigid(PefWorld* world, uint32_t n_bodies, const double* com, double* force,
+                               double* torque);
+
 /* ---- dam-break quantities of interest (validation.rs:63-117) --------------- */
 
 /* column_heights (validation.rs:66-92) of scene k: n_bins = max(1, ceil((x_max -

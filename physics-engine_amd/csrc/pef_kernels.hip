@@ -604,6 +604,95 @@ __global__ void k_column_counts(uint32_t first, uint32_t count, State cur, doubl
   if (b < n_bins) atomicAdd(&counts[b], 1u);
 }
 
+// ---------------------------------------------------------------------------
+// rigid-body samples (coupling.rs:289-331): FluidRigidCoupler::sync_from_rigid
+// and reduce_with_state on the device, FP64, no contraction (the reference's
+// operation order)
+// ---------------------------------------------------------------------------
+__global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const uint32_t* __restrict__ body,
+                             const double* __restrict__ local, const double* __restrict__ local_n,
+                             const double* __restrict__ pose, const uint32_t* __restrict__ slot_of_id,
+                             const SceneDev* __restrict__ sc, State cur, Ctl* ctlw) {
+#pragma clang fp contract(off)
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* P = pose + 7ull * body[k];  // x, y, z, qw, qx, qy, qz
+  const double w = P[3], x = P[4], y = P[5], z = P[6];
+  // Quat::to_matrix (quat.rs:119-138)
+  const double R[3][3] = {{1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)},
+                          {2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x)},
+                          {2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)}};
+  const double* l = local + 3ull * k;
+  const double* ln = local_n + 3ull * k;
+  double p[3], nn[3];
+  for (int a = 0; a < 3; ++a) {  // Quat::rotate (quat.rs:102-109), then x + r
+    p[a] = P[a] + (R[a][0] * l[0] + R[a][1] * l[1] + R[a][2] * l[2]);
+    nn[a] = R[a][0] * ln[0] + R[a][1] * ln[1] + R[a][2] * ln[2];
+  }
+  const uint32_t s = slot_of_id[id[k]];
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  if (S.dim == 2) {
+    p[2] = 0.0;
+    nn[2] = 0.0;
+  }
+  const double ox = cur.X[s], oy = cur.Y[s], oz = cur.Z[s];
+  cur.X[s] = p[0];
+  cur.Y[s] = p[1];
+  cur.Z[s] = p[2];
+  const float wgt = cur.A[s].w;
+  const float4 u = make_float4(__int_as_float(to_units(p[0], S.org[0], S.inv_unit)),
+                               __int_as_float(to_units(p[1], S.org[1], S.inv_unit)),
+                               __int_as_float(to_units(p[2], S.org[2], S.inv_unit)), wgt);
+  cur.A[s] = u;
+  cur.A2[s] = u;
+  cur.NX[s] = nn[0];
+  cur.NY[s] = nn[1];
+  cur.NZ[s] = nn[2];
+  const double dx = p[0] - ox, dy = p[1] - oy, dz = p[2] - oz;
+  const float d = cur.DISP[s] + (float)sqrt(dx * dx + dy * dy + dz * dz);
+  cur.DISP[s] = d;
+  if (!(d <= S.skin_half)) ctlw->need_rebuild = 1u;
+}
+
+// One workgroup per body: force = sum of the samples' boundary reactions,
+// torque = sum (x_s - com) x f_s, FP64. Each thread sums its strided share of
+// the body's samples in order; a fixed tree combines the threads. Reactions
+// sit in the slots of the last force evaluation (slot_of_idf), positions in
+// the current ones (slot_of_id).
+__global__ void k_rigid_reduce(const uint32_t* __restrict__ order, const uint32_t* __restrict__ body_start,
+                               const uint32_t* __restrict__ id, const uint32_t* __restrict__ slot_of_id,
+                               const uint32_t* __restrict__ slot_of_idf, State cur, const double* __restrict__ com,
+                               double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double red[6][256];
+  const uint32_t b = blockIdx.x;
+  const double cx = com[3 * b], cy = com[3 * b + 1], cz = com[3 * b + 2];
+  double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+  for (uint32_t k = body_start[b] + threadIdx.x; k < body_start[b + 1]; k += blockDim.x) {
+    const uint32_t i = id[order[k]], s = slot_of_id[i];
+    const float4 fb4 = cur.BF[slot_of_idf[i]];
+    const double fx = fb4.x, fy = fb4.y, fz = fb4.z;
+    const double rx = cur.X[s] - cx, ry = cur.Y[s] - cy, rz = cur.Z[s] - cz;
+    f[0] += fx;
+    f[1] += fy;
+    f[2] += fz;
+    t[0] += ry * fz - rz * fy;  // Vec3::cross (vec3.rs:87-93)
+    t[1] += rz * fx - rx * fz;
+    t[2] += rx * fy - ry * fx;
+  }
+  for (int c = 0; c < 3; ++c) {
+    red[c][threadIdx.x] = f[c];
+    red[3 + c][threadIdx.x] = t[c];
+  }
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[6ull * b + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // slot of each original index
 __global__ void k_slot_of_id(uint32_t N, State cur, uint32_t* __restrict__ slot_of_id) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -741,6 +830,20 @@ void launch_column_counts(hipStream_t st, uint32_t first, uint32_t count, const 
                           double x_max, double bw, uint64_t n_bins, uint32_t* counts) {
   if (count == 0) return;
   PEF_LAUNCH(k_column_counts, blocks_for(count, 256), 256, st, first, count, cur, x_min, x_max, bw, n_bins, counts);
+}
+
+void launch_rigid_sync(hipStream_t st, uint32_t n, const uint32_t* id, const uint32_t* body, const double* local,
+                       const double* local_n, const double* pose, const uint32_t* slot_of_id, const SceneDev* sc,
+                       const State& cur, Ctl* ctlw) {
+  if (n == 0) return;
+  PEF_LAUNCH(k_rigid_sync, blocks_for(n, 256), 256, st, n, id, body, local, local_n, pose, slot_of_id, sc, cur, ctlw);
+}
+
+void launch_rigid_reduce(hipStream_t st, uint32_t n_bodies, const uint32_t* order, const uint32_t* body_start,
+                         const uint32_t* id, const uint32_t* slot_of_id, const uint32_t* slot_of_idf,
+                         const State& cur, const double* com, double* out) {
+  if (n_bodies == 0) return;
+  PEF_LAUNCH(k_rigid_reduce, n_bodies, 256, st, order, body_start, id, slot_of_id, slot_of_idf, cur, com, out);
 }
 
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id) {

@@ -59,6 +59,16 @@ void launch_probe_candidates(hipStream_t st, const SceneDev* sc, uint32_t scene,
                              uint64_t* out_len);
 void launch_column_counts(hipStream_t st, uint32_t first, uint32_t count, const State& cur, double x_min,
                           double x_max, double bw, uint64_t n_bins, uint32_t* counts);
+// FluidRigidCoupler::sync_from_rigid for samples k < n: pose holds per body
+// x, y, z, qw, qx, qy, qz; positions, integer frame and normals of the
+// samples' slots are rewritten and their displacement feeds the skin test.
+void launch_rigid_sync(hipStream_t st, uint32_t n, const uint32_t* id, const uint32_t* body, const double* local,
+                       const double* local_n, const double* pose, const uint32_t* slot_of_id, const SceneDev* sc,
+                       const State& cur, Ctl* ctlw);
+// FluidRigidCoupler::reduce_with_state: out[6 b .. 6 b + 5] = force, torque of body b.
+void launch_rigid_reduce(hipStream_t st, uint32_t n_bodies, const uint32_t* order, const uint32_t* body_start,
+                         const uint32_t* id, const uint32_t* slot_of_id, const uint32_t* slot_of_idf,
+                         const State& cur, const double* com, double* out);
 void launch_slot_of_id(hipStream_t st, uint32_t N, const State& cur, uint32_t* slot_of_id);
 void launch_list_stats(hipStream_t st, uint32_t N, const uint32_t* CNT, Ctl* ctl);
 void launch_calib_copy(hipStream_t st, const float4* src, float4* dst, uint64_t n);

@@ -97,6 +97,12 @@ class World {
   uint64_t sur_k = 0;         // steps since attaching
   bool sur_fallback = false;  // the last check failed: plain force up to the next check
   PefSurrogateStats sur_stats{};
+  // rigid-body samples (set by pef_world_set_rigid_samples)
+  uint32_t rs_n = 0, rs_bodies = 0;
+  uint32_t *rs_id = nullptr, *rs_body = nullptr, *rs_order = nullptr, *rs_start = nullptr, *slot_map = nullptr,
+           *slot_map_f = nullptr;
+  double *rs_local = nullptr, *rs_localn = nullptr, *rs_pose = nullptr, *rs_out = nullptr;
+  void* rs_mem = nullptr;
 
   ~World() { release(); }
 
@@ -108,6 +114,8 @@ class World {
     return e;
   }
   void release() {
+    if (rs_mem) (void)hipFree(rs_mem);
+    rs_mem = nullptr;
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
     release_probe();
@@ -1170,6 +1178,118 @@ int32_t set_surrogate(World& w, PefMlp* mlp, uint32_t check_every, double accel_
   return PEF_OK;
 }
 
+// FluidRigidCoupler samples (coupling.rs:267-286): original particle index,
+// body, body-frame offset and outward normal of each, kept on the device with
+// the samples ordered by body for the reduction.
+int32_t set_rigid_samples(World& w, uint64_t n, const uint64_t* particle, const uint32_t* body, const double* local,
+                          const double* local_n, uint32_t n_bodies) {
+  if (n && (!particle || !body || !local || !local_n)) return PEF_ERR_NULL;
+  if (n > w.N || n_bodies > (1u << 20) || (n && n_bodies == 0)) return PEF_ERR_BOUNDS;
+  std::vector<uint32_t> ids(n), order(n), start(n_bodies + 1, 0);
+  for (uint64_t k = 0; k < n; ++k) {
+    if (particle[k] >= w.N || body[k] >= n_bodies) return PEF_ERR_BOUNDS;
+    if (w.h_kind[particle[k]] == PEF_FLUID) return PEF_ERR_ARG;
+    if (!finite3(local + 3 * k) || !finite3(local_n + 3 * k)) return PEF_ERR_ARG;
+    ids[k] = (uint32_t)particle[k];
+    start[body[k] + 1]++;
+  }
+  for (uint32_t b = 0; b < n_bodies; ++b) start[b + 1] += start[b];
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+  for (uint64_t k = 0; k < n; ++k) order[fill[body[k]]++] = (uint32_t)k;  // stable: sample order within a body
+  if (w.rs_mem) HIPTRY(hipFree(w.rs_mem));
+  w.rs_mem = nullptr;
+  w.rs_n = 0;
+  w.rs_bodies = 0;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_u32 = up(std::max<size_t>(n, 1) * 4), b_d3 = up(std::max<size_t>(n, 1) * 24);
+  const size_t b_start = up((n_bodies + 1) * 4), b_pose = up(std::max<uint32_t>(n_bodies, 1) * 7 * 8);
+  const size_t b_out = up(std::max<uint32_t>(n_bodies, 1) * 6 * 8), b_map = up((size_t)w.N * 4);
+  HIPTRY(hipMalloc(&w.rs_mem, 3 * b_u32 + 2 * b_d3 + b_start + b_pose + b_out + 2 * b_map));
+  char* p = static_cast<char*>(w.rs_mem);
+  auto take = [&p](size_t b) {
+    char* q = p;
+    p += b;
+    return q;
+  };
+  w.rs_id = reinterpret_cast<uint32_t*>(take(b_u32));
+  w.rs_body = reinterpret_cast<uint32_t*>(take(b_u32));
+  w.rs_order = reinterpret_cast<uint32_t*>(take(b_u32));
+  w.rs_local = reinterpret_cast<double*>(take(b_d3));
+  w.rs_localn = reinterpret_cast<double*>(take(b_d3));
+  w.rs_start = reinterpret_cast<uint32_t*>(take(b_start));
+  w.rs_pose = reinterpret_cast<double*>(take(b_pose));
+  w.rs_out = reinterpret_cast<double*>(take(b_out));
+  w.slot_map = reinterpret_cast<uint32_t*>(take(b_map));
+  w.slot_map_f = reinterpret_cast<uint32_t*>(take(b_map));
+  if (n) {
+    HIPTRY(hipMemcpyAsync(w.rs_id, ids.data(), n * 4, hipMemcpyHostToDevice, w.stream));
+    HIPTRY(hipMemcpyAsync(w.rs_body, body, n * 4, hipMemcpyHostToDevice, w.stream));
+    HIPTRY(hipMemcpyAsync(w.rs_order, order.data(), n * 4, hipMemcpyHostToDevice, w.stream));
+    HIPTRY(hipMemcpyAsync(w.rs_local, local, n * 24, hipMemcpyHostToDevice, w.stream));
+    HIPTRY(hipMemcpyAsync(w.rs_localn, local_n, n * 24, hipMemcpyHostToDevice, w.stream));
+  }
+  HIPTRY(hipMemcpyAsync(w.rs_start, start.data(), (n_bodies + 1) * 4, hipMemcpyHostToDevice, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  w.rs_n = (uint32_t)n;
+  w.rs_bodies = n_bodies;
+  return PEF_OK;
+}
+
+// sync_from_rigid (coupling.rs:289-305) for the pose (x, q) of each body, then
+// the lists (when the skin test asks), densities and pressures of the moved
+// state, as the reference's next step begins with rebuild_grid and
+// compute_density.
+int32_t sync_rigid(World& w, uint32_t n_bodies, const double* x, const double* q) {
+  if (!x || !q) return PEF_ERR_NULL;
+  if (n_bodies != w.rs_bodies) return PEF_ERR_BOUNDS;
+  if (w.rs_n == 0) return PEF_OK;
+  std::vector<double> pose((size_t)n_bodies * 7);
+  for (uint32_t b = 0; b < n_bodies; ++b) {
+    for (int a = 0; a < 3; ++a) pose[7 * b + a] = x[3 * b + a];
+    for (int a = 0; a < 4; ++a) pose[7 * b + 3 + a] = q[4 * b + a];
+    for (int a = 0; a < 7; ++a)
+      if (!std::isfinite(pose[7 * b + a])) return PEF_ERR_ARG;
+  }
+  int32_t rc = ensure_density(w);
+  if (rc) return rc;
+  HIPTRY(hipMemcpyAsync(w.rs_pose, pose.data(), pose.size() * 8, hipMemcpyHostToDevice, w.stream));
+  pef::launch_slot_of_id(w.stream, w.N, w.cur, w.slot_map);
+  pef::launch_rigid_sync(w.stream, w.rs_n, w.rs_id, w.rs_body, w.rs_local, w.rs_localn, w.rs_pose, w.slot_map,
+                         w.sc_d, w.cur, w.ctl_d);
+  enqueue_rebuild(w, nullptr);
+  launch_density(w);
+  HIPTRY(hipGetLastError());
+  w.probe_valid = false;
+  return read_flags(w);
+}
+
+// reduce_with_state (coupling.rs:308-331): force and torque about com of each
+// body from the samples' boundary reactions of the last force evaluation.
+int32_t reduce_rigid(World& w, uint32_t n_bodies, const double* com, double* force, double* torque) {
+  if (!com || !force || !torque) return PEF_ERR_NULL;
+  if (n_bodies != w.rs_bodies) return PEF_ERR_BOUNDS;
+  for (uint32_t b = 0; b < n_bodies; ++b)
+    if (!finite3(com + 3 * b)) return PEF_ERR_ARG;
+  if (n_bodies == 0) return PEF_OK;
+  std::vector<double> out((size_t)n_bodies * 6);
+  HIPTRY(hipMemcpyAsync(w.rs_pose, com, (size_t)n_bodies * 24, hipMemcpyHostToDevice, w.stream));
+  pef::launch_slot_of_id(w.stream, w.N, w.cur, w.slot_map);
+  pef::State at_force = w.cur;  // the reactions' slots: IDF names their particles
+  at_force.ID = w.cur.IDF;
+  pef::launch_slot_of_id(w.stream, w.N, at_force, w.slot_map_f);
+  pef::launch_rigid_reduce(w.stream, n_bodies, w.rs_order, w.rs_start, w.rs_id, w.slot_map, w.slot_map_f, w.cur,
+                           w.rs_pose, w.rs_out);
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipMemcpyAsync(out.data(), w.rs_out, out.size() * 8, hipMemcpyDeviceToHost, w.stream));
+  HIPTRY(hipStreamSynchronize(w.stream));
+  for (uint32_t b = 0; b < n_bodies; ++b)
+    for (int a = 0; a < 3; ++a) {
+      force[3 * b + a] = out[6 * b + a];
+      torque[3 * b + a] = out[6 * b + 3 + a];
+    }
+  return PEF_OK;
+}
+
 int32_t set_state(World& w, const double* x, const double* v, const double* psi, const double* n_out) {
   const uint64_t N = w.N;
   for (uint64_t i = 0; i < N; ++i) {
@@ -1578,6 +1698,23 @@ int32_t pef_world_set_surrogate(PefWorld* world, PefMlp* mlp, uint32_t check_eve
                                 double density_tol) {
   if (!world) return PEF_ERR_NULL;
   return guard([&] { return set_surrogate(world->w, mlp, check_every, accel_tol, density_tol); });
+}
+
+int32_t pef_world_set_rigid_samples(PefWorld* world, uint64_t n, const uint64_t* particle, const uint32_t* body,
+                                    const double* local, const double* local_n, uint32_t n_bodies) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return set_rigid_samples(world->w, n, particle, body, local, local_n, n_bodies); });
+}
+
+int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return sync_rigid(world->w, n_bodies, x, q); });
+}
+
+int32_t pef_world_reduce_rigid(PefWorld* world, uint32_t n_bodies, const double* com, double* force,
+                               double* torque) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return reduce_rigid(world->w, n_bodies, com, force, torque); });
 }
 
 int32_t pef_world_surrogate_stats(PefWorld* world, PefSurrogateStats* out) {

@@ -185,6 +185,9 @@ def lib():
         "pef_world_get_features": (i32, [vp, vp, u64]),
         "pef_world_sample": (i32, [vp, vp, u64, u64]),
         "pef_world_set_surrogate": (i32, [vp, vp, u32, d, d]),
+        "pef_world_set_rigid_samples": (i32, [vp, u64, vp, vp, vp, vp, u32]),
+        "pef_world_sync_rigid": (i32, [vp, u32, vp, vp]),
+        "pef_world_reduce_rigid": (i32, [vp, u32, vp, vp, vp]),
         "pef_world_surrogate_stats": (i32, [vp, P(PefSurrogateStats)]),
     }
     for name, (res, args) in sig.items():
@@ -581,6 +584,29 @@ class WcsphWorld:
                       C.POINTER(C.c_double)]
         _check(f(self._h, scene, dx, h0, channel_length, h_frac, C.byref(out)), "front_x_height")
         return out.value
+
+    # --- rigid-body samples (coupling.rs:267-331) ---
+    def set_rigid_samples(self, particle, body, local, local_n, n_bodies: int):
+        particle = np.ascontiguousarray(particle, dtype=np.uint64)
+        body = np.ascontiguousarray(body, dtype=np.uint32)
+        local = np.ascontiguousarray(local, dtype=np.float64).reshape(-1, 3)
+        local_n = np.ascontiguousarray(local_n, dtype=np.float64).reshape(-1, 3)
+        _check(lib().pef_world_set_rigid_samples(self._h, len(particle), _ptr(particle), _ptr(body), _ptr(local),
+                                                 _ptr(local_n), n_bodies), "set_rigid_samples")
+        self._rigid_bodies = n_bodies
+
+    def sync_rigid(self, x, q):
+        """sync_from_rigid: body positions [nb][3] and quaternions [nb][4] (w, x, y, z)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 4)
+        _check(lib().pef_world_sync_rigid(self._h, len(x), _ptr(x), _ptr(q)), "sync_rigid")
+
+    def reduce_rigid(self, com):
+        """reduce_with_state: (force [nb][3], torque [nb][3]) about the bodies' centres."""
+        com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
+        f, t = np.zeros_like(com), np.zeros_like(com)
+        _check(lib().pef_world_reduce_rigid(self._h, len(com), _ptr(com), _ptr(f), _ptr(t)), "reduce_rigid")
+        return f, t
 
     # --- learned force surrogate ---
     def features(self) -> np.ndarray:

@@ -1,0 +1,152 @@
+"""The coupler's side of the WCSPH step on the device (coupling.rs:267-331).
+
+Bars:
+  - sync_from_rigid places every sample at x + R(q) local with normal
+    R(q) local_n bit for bit as the reference's FP64 operation order gives it
+    (restated here with numpy, which rounds every operation like the device
+    without contraction); rotations keep every |x_s - x_b| = |local| to 1 ulp
+    scale;
+  - reduce_with_state equals the FP64 sums of the device's own boundary_force
+    over each body's samples, to 1e-12 relative, also after list rebuilds
+    permuted the slots, and per body;
+  - a translating one-sided obstacle driven through sync_rigid follows the f64
+    oracle driven through set_state: density RMS relative <= 1e-5 every five
+    steps, boundary reactions and the reduced force within 1e-4.
+"""
+import numpy as np
+import pytest
+
+import pef
+import pyoracle as O
+from pef import scenes
+from helpers import rel_rms, vec_rel_rms
+from test_gpu_parity import _box_surface
+
+pytestmark = pytest.mark.gpu
+
+CFG = scenes.small(16, 8, 10, dx=0.01, steps=30)
+
+
+def _quat(axis, angle):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.concatenate([[np.cos(angle / 2)], np.sin(angle / 2) * axis])
+
+
+def _rotate(q, v):
+    """Quat::rotate (quat.rs:102-138) in numpy, the reference's operation order."""
+    w, x, y, z = q
+    R = [[1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)],
+         [2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x)],
+         [2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)]]
+    v = np.asarray(v, float)
+    return np.stack([R[a][0] * v[:, 0] + R[a][1] * v[:, 1] + R[a][2] * v[:, 2] for a in range(3)], 1)
+
+
+def _scene():
+    """The small dam break with a one-sided box obstacle; (params, particles,
+    obstacle ids, obstacle centre, local offsets, local normals)."""
+    dx, h = CFG.dx, CFG.h
+    m = O.mass_calibrated_to_rest(CFG.rest_density, dx, h, 3)
+    o = CFG.wall_gap * dx
+    P = O.Particles(3, CFG.rest_density, m, h)
+    P.fill_box((o, o, o), CFG.nx, CFG.ny, CFG.nz, dx)
+    P.jitter(0, 0.01 * dx)
+    P.fill_box_boundary((0.0, 0.0, 0.0), CFG.tank, dx, 1)
+    a = P.arrays()
+    lo, hi = np.array([0.19, 0.03, 0.03]), np.array([0.23, 0.07, 0.07])
+    obs_x, obs_n = _box_surface(lo, hi, dx)
+    x = np.concatenate([a["x"], obs_x])
+    kind = np.concatenate([a["kind"], np.ones(len(obs_x), np.uint8)])
+    n_out = np.concatenate([np.zeros_like(a["x"]), obs_n])
+    psi = np.concatenate([a["psi"], np.full(len(obs_x), dx ** 3)])
+    G = pef.Particles(3, CFG.rest_density, m, h)
+    G._append(x, np.zeros_like(x), kind, psi, n_out)
+    pef.recomputed_akinci_psi(G)
+    params = pef.WcsphParams.water(CFG.rest_density, CFG.sound_speed, CFG.gravity)
+    params.visc_alpha = CFG.visc_alpha
+    params.adapt_sound_speed = False
+    ids = np.arange(len(a["x"]), len(x))
+    c = 0.5 * (lo + hi)
+    return params, G, ids, c, obs_x - c, obs_n
+
+
+def _world():
+    params, G, ids, c, local, local_n = _scene()
+    W = pef.WcsphWorld(G, params)
+    W.set_rigid_samples(ids, np.zeros(len(ids), np.uint32), local, local_n, 1)
+    return W, G, ids, c, local, local_n
+
+
+def test_sync_places_samples_like_the_coupler():
+    W, G, ids, c, local, local_n = _world()
+    x_b = c + np.array([0.004, -0.002, 0.003])
+    q = _quat([0.3, -0.5, 1.0], 0.4)
+    W.sync_rigid(x_b[None], q[None])
+    xs = W.positions()[ids]
+    want = x_b + _rotate(q, local)
+    assert np.array_equal(xs.view(np.uint64), want.view(np.uint64))
+    d = np.linalg.norm(xs - x_b, axis=1)
+    assert np.allclose(d, np.linalg.norm(local, axis=1), rtol=1e-15, atol=1e-17)
+    # the identity pose puts the samples back where they were built
+    W.sync_rigid(c[None], np.array([[1.0, 0.0, 0.0, 0.0]]))
+    assert np.array_equal(W.positions()[ids], c + local)
+    # the rest of the world is untouched
+    others = np.setdiff1d(np.arange(W.n), ids)
+    assert np.array_equal(W.positions()[others], G.x[others])
+
+
+def test_reduce_sums_the_reactions_per_body():
+    W, G, ids, c, local, local_n = _world()
+    body = (np.arange(len(ids)) % 2).astype(np.uint32)  # interleaved: both bodies touch the fluid
+    W.set_rigid_samples(ids, body, local, local_n, 2)
+    vel = np.array([-0.2, 0.0, 0.05])
+    for k in range(CFG.steps):
+        W.sync_rigid((c + (k + 1) * CFG.dt * vel)[None].repeat(2, 0), np.tile([1.0, 0.0, 0.0, 0.0], (2, 1)))
+        W.step(CFG.dt)
+    _, rebuilds = W.list_stats()
+    assert rebuilds > 1  # slots were permuted between force evaluations
+    com = np.array([c + CFG.steps * CFG.dt * vel, c])
+    f, t = W.reduce_rigid(com)
+    bf, x = W.boundary_force(), W.positions()
+    for b, sel in enumerate((ids[0::2], ids[1::2])):
+        fw = bf[sel].sum(0)
+        tw = np.cross(x[sel] - com[b], bf[sel]).sum(0)
+        assert np.abs(fw).sum() > 0
+        assert np.allclose(f[b], fw, rtol=1e-12, atol=1e-12 * np.abs(fw).max())
+        assert np.allclose(t[b], tw, rtol=1e-12, atol=1e-12 * np.abs(tw).max())
+
+
+def test_translating_obstacle_through_sync_matches_oracle():
+    W, G, ids, c, local, local_n = _world()
+    prm = dict(dim=3, rest_density=CFG.rest_density, mass=G.mass, h=G.h, sound_speed=CFG.sound_speed,
+               visc_alpha=CFG.visc_alpha, gravity=CFG.gravity, adapt_sound_speed=False)
+    Wo = O.World(prm, G.x, G.kind, psi=G.psi, n_out=G.n_out, threads=4)
+    vel = np.array([-0.2, 0.0, 0.05])
+    fl = G.kind == pef.FLUID
+    for k in range(CFG.steps):
+        x_b = c + (k + 1) * CFG.dt * vel
+        W.sync_rigid(x_b[None], np.array([[1.0, 0.0, 0.0, 0.0]]))
+        so = Wo.state()
+        xo, vo = so["x"].reshape(-1, 3).copy(), so["v"].reshape(-1, 3).copy()
+        xo[ids], vo[ids] = x_b + local, vel
+        Wo.set_state(x=xo, v=vo)
+        W.step(CFG.dt)
+        Wo.step(CFG.dt)
+        if k % 5 == 4:
+            e = rel_rms(W.densities()[fl], Wo.state()["rho"][fl])
+            assert e <= 1e-5, (k, e)
+    bf, bfo = W.boundary_force(), Wo.state()["boundary_force"].reshape(-1, 3)
+    assert vec_rel_rms(bf[ids], bfo[ids]) <= 1e-4
+    f, _ = W.reduce_rigid((c + CFG.steps * CFG.dt * vel)[None])
+    fo = bfo[ids].sum(0)
+    assert np.linalg.norm(f[0] - fo) <= 1e-4 * np.linalg.norm(fo)
+
+
+def test_rigid_sample_errors():
+    W, G, ids, c, local, local_n = _world()
+    with pytest.raises(pef.PefError):  # a fluid particle cannot be a sample
+        W.set_rigid_samples([0], [0], local[:1], local_n[:1], 1)
+    with pytest.raises(pef.PefError):
+        W.sync_rigid(np.zeros((2, 3)), np.tile([1.0, 0, 0, 0], (2, 1)))  # body count differs
+    with pytest.raises(pef.PefError):
+        W.sync_rigid(np.array([[np.nan, 0, 0]]), np.array([[1.0, 0, 0, 0]]))
