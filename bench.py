@@ -149,6 +149,8 @@ def main():
                     help="trained surrogate (.npz); default: the committed network of the configuration, if any")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--check-every", type=int, default=10, help="surrogate: plain-force check every k steps")
+    ap.add_argument("--coupled-steps", type=int, default=100,
+                    help="obstacle configurations: steps with the obstacle moved through the device coupler")
     ap.add_argument("--accel-tol", type=float, default=0.2,
                     help="surrogate: fall back when a check's acceleration error exceeds this")
     args = ap.parse_args()
@@ -268,6 +270,8 @@ def main():
         assert pef.lib().pefx_calibration_copy(args.calibration_copy_mib << 20) == 0
     if tune is not None:
         out["autotune"] = tune
+    if parts is not None and cfg.obstacle is not None and args.coupled_steps > 0:
+        out["coupled"] = coupled_leg(W, parts, cfg, args.coupled_steps, ms_per_step)
     # the network trained on the 1M dam break serves every 3-D dam break at its spacing
     sur_path = args.surrogate or os.path.join(ROOT, "physics-engine_amd", "pef", "models",
                                               "surrogate_dam_break_1m.npz")
@@ -281,6 +285,58 @@ def main():
     if not args.no_cpu_baseline and parts is not None and cfg.n_fluid <= scenes.GPU_1M.n_fluid:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))
+
+
+def coupled_leg(W, parts, cfg, steps, ms_plain):
+    """The coupling caller's step (coupling.rs:853-865) with the obstacle as one
+    rigid body moved by a prescribed heave: FluidRigidCoupler sync and reduce
+    on the device (pef_world_sync_rigid / pef_world_reduce_rigid) around each
+    fluid step, against the same motion through the host state round trip
+    (pef_world_set_state + boundary_force) for a few steps."""
+    import numpy as np
+    import pef
+    from pef import scenes
+
+    # the obstacle's samples are the last ones the scene appends
+    n_obs = scenes.sample_box(pef.Particles(parts.dim, parts.rest_density, parts.mass, parts.h),
+                              cfg.obstacle[0], cfg.obstacle[1], cfg.dx)
+    ids = np.arange(len(parts) - n_obs, len(parts))
+    c = np.asarray(cfg.obstacle[0], float)
+    local, local_n = parts.x[ids] - c, parts.n_out[ids]
+    W.set_rigid_samples(ids, np.zeros(n_obs, np.uint32), local, local_n, 1)
+    q = np.array([[1.0, 0.0, 0.0, 0.0]])
+
+    om = 2 * np.pi * 20.0
+
+    def pose(k):  # 1 mm heave at 20 Hz
+        return (c + np.array([0.0, 0.0, 1e-3 * np.sin(om * k * cfg.dt)]))[None]
+
+    def vel(k):
+        return np.array([[0.0, 0.0, 1e-3 * om * np.cos(om * k * cfg.dt)]])
+
+    t0 = time.perf_counter()
+    for k in range(steps):
+        W.sync_rigid(pose(k), q, vel(k))
+        W.step(cfg.dt)
+        f, t = W.reduce_rigid(pose(k))
+    dev = (time.perf_counter() - t0) / steps
+    host_steps = 3
+    t0 = time.perf_counter()
+    for k in range(steps, steps + host_steps):
+        x = W.positions()
+        x[ids] = pose(k) + local
+        v = W.velocities()
+        v[ids] = vel(k)
+        W.set_state(x=x, v=v)
+        W.step(cfg.dt)
+        bf = W.boundary_force()
+        _ = bf[ids].sum(0)
+    host = (time.perf_counter() - t0) / host_steps
+    print(f"# coupled: device coupler {1e3 * dev:.3f} ms/step, host round trip {1e3 * host:.1f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"steps": steps, "samples": n_obs, "ms_per_step_device_coupler": round(1e3 * dev, 4),
+            "ms_per_step_plain": round(ms_plain, 4), "ms_per_step_host_round_trip": round(1e3 * host, 3),
+            "last_force_N": [round(v, 6) for v in f[0]], "last_torque_Nm": [round(v, 6) for v in t[0]]}
 
 
 def surrogate_leg(cfg, opts, path, rho_plain, fluid, n_fluid, args, value_plain):

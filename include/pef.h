@@ -188,6 +188,11 @@ int32_t pef_world_get_accelerations(PefWorld* world, double* out, uint64_t len);
 int32_t pef_world_get_densities(PefWorld* world, double* out, uint64_t len);
 int32_t pef_world_get_pressures(PefWorld* world, double* out, uint64_t len);
 int32_t pef_world_get_boundary_force(PefWorld* world, double* out, uint64_t len);
+/* particles.rs:41-47 fields psi (n doubles, as created or last set) and n_out
+ * (3n doubles: the outward normals of one-sided samples, as sync_rigid last
+ * turned them; zero for every other particle). */
+int32_t pef_world_get_psi(PefWorld* world, double* out, uint64_t len);
+int32_t pef_world_get_normals(PefWorld* world, double* out, uint64_t len);
 
 /* Setters used by coupling callers between steps (coupling.rs:289-305 writes
  * boundary x, v, n_out each step); any pointer may be NULL; arrays are 3n or n
@@ -247,13 +252,16 @@ int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* re
 int32_t pef_world_set_rigid_samples(PefWorld* world, uint64_t n, const uint64_t* particle, const uint32_t* body,
                                     const double* local, const double* local_n, uint32_t n_bodies);
 /* FluidRigidCoupler::sync_from_rigid (coupling.rs:289-305) on the device:
- * body b at position x[3b..] and unit quaternion q[4b..] = (w, x, y, z) puts
- * its samples at x + R(q) local with normals R(q) local_n (FP64, the
- * reference's operation order; 2-D zeroes z). Then, as the reference's next
- * step begins, lists (when the skin test asks), densities and pressures are
- * brought up to date. Sample velocities are not taken: the WCSPH step never
- * reads a boundary particle's velocity (wcsph.rs:117-323). */
-int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q);
+ * body b at position x[3b..], unit quaternion q[4b..] = (w, x, y, z), velocity
+ * v[3b..] and world-frame angular velocity omega[3b..] puts its samples at
+ * x + r, r = R(q) local, with velocity v + omega x r and normal R(q) local_n
+ * (FP64, the reference's operation order; velocities stored FP32; 2-D zeroes
+ * z). v and omega may be NULL: the body is at rest. The WCSPH step never
+ * reads a boundary particle's velocity (wcsph.rs:117-323); it is kept for
+ * readers of the state. Then, as the reference's next step begins, lists
+ * (when the skin test asks), densities and pressures are brought up to date. */
+int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q, const double* v,
+                             const double* omega);
 /* FluidRigidCoupler::reduce_with_state (coupling.rs:308-331) on the device:
  * force[3b..] = sum of the samples' boundary_force and torque[3b..] = sum of
  * (x_s - com_b) x f_s, FP64 with a fixed summation tree, from the last force

@@ -616,7 +616,7 @@ __global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const 
 #pragma clang fp contract(off)
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double* P = pose + 7ull * body[k];  // x, y, z, qw, qx, qy, qz
+  const double* P = pose + 13ull * body[k];  // x, y, z, qw, qx, qy, qz, vx, vy, vz, wx, wy, wz
   const double w = P[3], x = P[4], y = P[5], z = P[6];
   // Quat::to_matrix (quat.rs:119-138)
   const double R[3][3] = {{1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)},
@@ -624,16 +624,23 @@ __global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const 
                           {2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)}};
   const double* l = local + 3ull * k;
   const double* ln = local_n + 3ull * k;
-  double p[3], nn[3];
+  double r[3], p[3], nn[3], vs[3];
   for (int a = 0; a < 3; ++a) {  // Quat::rotate (quat.rs:102-109), then x + r
-    p[a] = P[a] + (R[a][0] * l[0] + R[a][1] * l[1] + R[a][2] * l[2]);
+    r[a] = R[a][0] * l[0] + R[a][1] * l[1] + R[a][2] * l[2];
+    p[a] = P[a] + r[a];
     nn[a] = R[a][0] * ln[0] + R[a][1] * ln[1] + R[a][2] * ln[2];
   }
+  // v + omega x r (vec3.rs:87-93)
+  const double* om = P + 10;
+  vs[0] = P[7] + (om[1] * r[2] - om[2] * r[1]);
+  vs[1] = P[8] + (om[2] * r[0] - om[0] * r[2]);
+  vs[2] = P[9] + (om[0] * r[1] - om[1] * r[0]);
   const uint32_t s = slot_of_id[id[k]];
   const SceneDev& S = sc[scene_of(cur.META[s])];
   if (S.dim == 2) {
     p[2] = 0.0;
     nn[2] = 0.0;
+    vs[2] = 0.0;
   }
   const double ox = cur.X[s], oy = cur.Y[s], oz = cur.Z[s];
   cur.X[s] = p[0];
@@ -648,6 +655,10 @@ __global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const 
   cur.NX[s] = nn[0];
   cur.NY[s] = nn[1];
   cur.NZ[s] = nn[2];
+  // the sample's velocity, kept for readers of the state: the step itself
+  // never reads a boundary particle's velocity
+  cur.B[s] = make_float4((float)vs[0], (float)vs[1], (float)vs[2], cur.B[s].w);
+  cur.B2[s] = make_float4((float)vs[0], (float)vs[1], (float)vs[2], cur.B2[s].w);
   const double dx = p[0] - ox, dy = p[1] - oy, dz = p[2] - oz;
   const float d = cur.DISP[s] + (float)sqrt(dx * dx + dy * dy + dz * dz);
   cur.DISP[s] = d;

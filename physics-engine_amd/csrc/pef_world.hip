@@ -1202,7 +1202,7 @@ int32_t set_rigid_samples(World& w, uint64_t n, const uint64_t* particle, const 
   w.rs_bodies = 0;
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t b_u32 = up(std::max<size_t>(n, 1) * 4), b_d3 = up(std::max<size_t>(n, 1) * 24);
-  const size_t b_start = up((n_bodies + 1) * 4), b_pose = up(std::max<uint32_t>(n_bodies, 1) * 7 * 8);
+  const size_t b_start = up((n_bodies + 1) * 4), b_pose = up(std::max<uint32_t>(n_bodies, 1) * 13 * 8);
   const size_t b_out = up(std::max<uint32_t>(n_bodies, 1) * 6 * 8), b_map = up((size_t)w.N * 4);
   HIPTRY(hipMalloc(&w.rs_mem, 3 * b_u32 + 2 * b_d3 + b_start + b_pose + b_out + 2 * b_map));
   char* p = static_cast<char*>(w.rs_mem);
@@ -1235,20 +1235,26 @@ int32_t set_rigid_samples(World& w, uint64_t n, const uint64_t* particle, const 
   return PEF_OK;
 }
 
-// sync_from_rigid (coupling.rs:289-305) for the pose (x, q) of each body, then
-// the lists (when the skin test asks), densities and pressures of the moved
-// state, as the reference's next step begins with rebuild_grid and
-// compute_density.
-int32_t sync_rigid(World& w, uint32_t n_bodies, const double* x, const double* q) {
+// sync_from_rigid (coupling.rs:289-305) for the pose (x, q) and motion (v,
+// omega; null = at rest) of each body, then the lists (when the skin test
+// asks), densities and pressures of the moved state, as the reference's next
+// step begins with rebuild_grid and compute_density.
+int32_t sync_rigid(World& w, uint32_t n_bodies, const double* x, const double* q, const double* v,
+                   const double* omega) {
   if (!x || !q) return PEF_ERR_NULL;
   if (n_bodies != w.rs_bodies) return PEF_ERR_BOUNDS;
   if (w.rs_n == 0) return PEF_OK;
-  std::vector<double> pose((size_t)n_bodies * 7);
+  std::vector<double> pose((size_t)n_bodies * 13, 0.0);
   for (uint32_t b = 0; b < n_bodies; ++b) {
-    for (int a = 0; a < 3; ++a) pose[7 * b + a] = x[3 * b + a];
-    for (int a = 0; a < 4; ++a) pose[7 * b + 3 + a] = q[4 * b + a];
-    for (int a = 0; a < 7; ++a)
-      if (!std::isfinite(pose[7 * b + a])) return PEF_ERR_ARG;
+    double* P = pose.data() + 13 * (size_t)b;
+    for (int a = 0; a < 3; ++a) P[a] = x[3 * b + a];
+    for (int a = 0; a < 4; ++a) P[3 + a] = q[4 * b + a];
+    for (int a = 0; a < 3; ++a) {
+      if (v) P[7 + a] = v[3 * b + a];
+      if (omega) P[10 + a] = omega[3 * b + a];
+    }
+    for (int a = 0; a < 13; ++a)
+      if (!std::isfinite(P[a])) return PEF_ERR_ARG;
   }
   int32_t rc = ensure_density(w);
   if (rc) return rc;
@@ -1302,6 +1308,9 @@ int32_t set_state(World& w, const double* x, const double* v, const double* psi,
   std::vector<double> hx(3 * N), hv(3 * N);
   int32_t rc = get_field(w, 0, hx.data(), hx.size(), true);
   if (!rc) rc = get_field(w, 1, hv.data(), hv.size(), true);
+  if (rc) return rc;
+  // normals move on the device with rigid samples (sync_rigid): keep them
+  if (!rc && !n_out) rc = get_field(w, 7, w.h_nout.data(), w.h_nout.size(), true);
   if (rc) return rc;
   if (x) std::memcpy(hx.data(), x, 3 * N * 8);
   if (v) std::memcpy(hv.data(), v, 3 * N * 8);
@@ -1660,7 +1669,19 @@ PEF_GETTER(pef_world_get_accelerations, 2, true)
 PEF_GETTER(pef_world_get_densities, 3, false)
 PEF_GETTER(pef_world_get_pressures, 4, false)
 PEF_GETTER(pef_world_get_boundary_force, 5, true)
+PEF_GETTER(pef_world_get_normals, 7, true)
 #undef PEF_GETTER
+
+int32_t pef_world_get_psi(PefWorld* world, double* out, uint64_t len) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] {
+    World& w = world->w;
+    if (!out) return (int32_t)PEF_ERR_NULL;
+    if (len < (uint64_t)w.N) return (int32_t)PEF_ERR_BOUNDS;
+    std::memcpy(out, w.h_psi.data(), (size_t)w.N * 8);
+    return (int32_t)PEF_OK;
+  });
+}
 
 int32_t pef_world_set_state(PefWorld* world, const double* x, const double* v, const double* psi,
                             const double* n_out) {
@@ -1706,9 +1727,10 @@ int32_t pef_world_set_rigid_samples(PefWorld* world, uint64_t n, const uint64_t*
   return guard([&] { return set_rigid_samples(world->w, n, particle, body, local, local_n, n_bodies); });
 }
 
-int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q) {
+int32_t pef_world_sync_rigid(PefWorld* world, uint32_t n_bodies, const double* x, const double* q, const double* v,
+                             const double* omega) {
   if (!world) return PEF_ERR_NULL;
-  return guard([&] { return sync_rigid(world->w, n_bodies, x, q); });
+  return guard([&] { return sync_rigid(world->w, n_bodies, x, q, v, omega); });
 }
 
 int32_t pef_world_reduce_rigid(PefWorld* world, uint32_t n_bodies, const double* com, double* force,

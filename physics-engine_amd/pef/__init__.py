@@ -152,6 +152,8 @@ def lib():
         "pef_world_get_densities": (i32, [vp, vp, u64]),
         "pef_world_get_pressures": (i32, [vp, vp, u64]),
         "pef_world_get_boundary_force": (i32, [vp, vp, u64]),
+        "pef_world_get_psi": (i32, [vp, vp, u64]),
+        "pef_world_get_normals": (i32, [vp, vp, u64]),
         "pef_world_set_state": (i32, [vp, vp, vp, vp, vp]),
         "pef_world_get_grid": (i32, [vp, vp, vp, vp, vp, u64, P(u64)]),
         "pef_world_get_neighbours": (i32, [vp, u64, vp, u64, P(u64)]),
@@ -186,7 +188,7 @@ def lib():
         "pef_world_sample": (i32, [vp, vp, u64, u64]),
         "pef_world_set_surrogate": (i32, [vp, vp, u32, d, d]),
         "pef_world_set_rigid_samples": (i32, [vp, u64, vp, vp, vp, vp, u32]),
-        "pef_world_sync_rigid": (i32, [vp, u32, vp, vp]),
+        "pef_world_sync_rigid": (i32, [vp, u32, vp, vp, vp, vp]),
         "pef_world_reduce_rigid": (i32, [vp, u32, vp, vp, vp]),
         "pef_world_surrogate_stats": (i32, [vp, P(PefSurrogateStats)]),
     }
@@ -461,6 +463,12 @@ class WcsphWorld:
     def boundary_force(self):
         return self._get("pef_world_get_boundary_force", True)
 
+    def psi(self):
+        return self._get("pef_world_get_psi", False)
+
+    def normals(self):
+        return self._get("pef_world_get_normals", True)
+
     def set_state(self, x=None, v=None, psi=None, n_out=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (x, v, psi, n_out)]
         _check(lib().pef_world_set_state(self._h, *[_ptr(a) for a in arrs]), "pef_world_set_state")
@@ -595,11 +603,18 @@ class WcsphWorld:
                                                  _ptr(local_n), n_bodies), "set_rigid_samples")
         self._rigid_bodies = n_bodies
 
-    def sync_rigid(self, x, q):
-        """sync_from_rigid: body positions [nb][3] and quaternions [nb][4] (w, x, y, z)."""
+    def sync_rigid(self, x, q, v=None, omega=None):
+        """sync_from_rigid: body positions [nb][3], quaternions [nb][4] (w, x, y, z)
+        and, optionally, velocities and world-frame angular velocities [nb][3]
+        (None: at rest)."""
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 4)
-        _check(lib().pef_world_sync_rigid(self._h, len(x), _ptr(x), _ptr(q)), "sync_rigid")
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        omega = None if omega is None else np.ascontiguousarray(omega, dtype=np.float64).reshape(-1, 3)
+        for a, name in ((v, "v"), (omega, "omega")):
+            if a is not None and len(a) != len(x):
+                raise ValueError(f"sync_rigid: {name} has {len(a)} rows for {len(x)} bodies")
+        _check(lib().pef_world_sync_rigid(self._h, len(x), _ptr(x), _ptr(q), _ptr(v), _ptr(omega)), "sync_rigid")
 
     def reduce_rigid(self, com):
         """reduce_with_state: (force [nb][3], torque [nb][3]) about the bodies' centres."""

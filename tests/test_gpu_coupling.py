@@ -2,10 +2,12 @@
 
 Bars:
   - sync_from_rigid places every sample at x + R(q) local with normal
-    R(q) local_n bit for bit as the reference's FP64 operation order gives it
-    (restated here with numpy, which rounds every operation like the device
-    without contraction); rotations keep every |x_s - x_b| = |local| to 1 ulp
-    scale;
+    R(q) local_n and velocity v + omega x (R(q) local) bit for bit as the
+    reference's FP64 operation order gives it (restated here with numpy, which
+    rounds every operation like the device without contraction; velocities
+    as their FP32 rounding); rotations keep every |x_s - x_b| = |local| to
+    1 ulp scale, and the sample velocities are a rigid field:
+    (v_s - v_b) . (x_s - x_b) = 0;
   - reduce_with_state equals the FP64 sums of the device's own boundary_force
     over each body's samples, to 1e-12 relative, also after list rebuilds
     permuted the slots, and per body;
@@ -77,22 +79,50 @@ def _world():
     return W, G, ids, c, local, local_n
 
 
+def _cross(a, b):
+    """Vec3::cross (vec3.rs:87-93), row-wise, the reference's operation order."""
+    return np.stack([a[..., 1] * b[:, 2] - a[..., 2] * b[:, 1], a[..., 2] * b[:, 0] - a[..., 0] * b[:, 2],
+                     a[..., 0] * b[:, 1] - a[..., 1] * b[:, 0]], 1)
+
+
 def test_sync_places_samples_like_the_coupler():
     W, G, ids, c, local, local_n = _world()
     x_b = c + np.array([0.004, -0.002, 0.003])
     q = _quat([0.3, -0.5, 1.0], 0.4)
-    W.sync_rigid(x_b[None], q[None])
+    v_b, w_b = np.array([0.3, -0.1, 0.02]), np.array([1.5, -4.0, 2.5])
+    W.sync_rigid(x_b[None], q[None], v_b[None], w_b[None])
     xs = W.positions()[ids]
-    want = x_b + _rotate(q, local)
-    assert np.array_equal(xs.view(np.uint64), want.view(np.uint64))
+    r = _rotate(q, local)
+    assert np.array_equal(xs.view(np.uint64), (x_b + r).view(np.uint64))
+    assert np.array_equal(W.normals()[ids], _rotate(q, local_n))
+    vs = W.velocities()[ids]
+    assert np.array_equal(vs, (v_b + _cross(w_b, r)).astype(np.float32).astype(np.float64))
+    assert np.abs(np.sum((vs - v_b) * r, 1)).max() <= 1e-6 * np.abs(vs - v_b).max() * np.abs(r).max()
     d = np.linalg.norm(xs - x_b, axis=1)
     assert np.allclose(d, np.linalg.norm(local, axis=1), rtol=1e-15, atol=1e-17)
-    # the identity pose puts the samples back where they were built
+    # the identity pose at rest puts the samples back where they were built
     W.sync_rigid(c[None], np.array([[1.0, 0.0, 0.0, 0.0]]))
     assert np.array_equal(W.positions()[ids], c + local)
+    assert not W.velocities()[ids].any()
     # the rest of the world is untouched
     others = np.setdiff1d(np.arange(W.n), ids)
     assert np.array_equal(W.positions()[others], G.x[others])
+    assert not W.velocities()[others].any()
+    assert np.array_equal(W.normals(), G.n_out) and np.array_equal(W.psi(), G.psi)
+
+
+def test_set_state_keeps_the_turned_normals():
+    """A host state write after sync_rigid (a caller mixing both) keeps the
+    normals the device turned, and the step then sees the turned samples."""
+    W, G, ids, c, local, local_n = _world()
+    q = _quat([0.0, 0.0, 1.0], 0.7)
+    W.sync_rigid(c[None], q[None])
+    turned = W.normals()
+    v = W.velocities()
+    v[G.kind == pef.FLUID] += [0.01, 0.0, 0.0]
+    W.set_state(v=v)
+    assert np.array_equal(W.normals(), turned)
+    assert np.array_equal(W.velocities(), v.astype(np.float32).astype(np.float64))
 
 
 def test_reduce_sums_the_reactions_per_body():
@@ -125,7 +155,7 @@ def test_translating_obstacle_through_sync_matches_oracle():
     fl = G.kind == pef.FLUID
     for k in range(CFG.steps):
         x_b = c + (k + 1) * CFG.dt * vel
-        W.sync_rigid(x_b[None], np.array([[1.0, 0.0, 0.0, 0.0]]))
+        W.sync_rigid(x_b[None], np.array([[1.0, 0.0, 0.0, 0.0]]), vel[None])
         so = Wo.state()
         xo, vo = so["x"].reshape(-1, 3).copy(), so["v"].reshape(-1, 3).copy()
         xo[ids], vo[ids] = x_b + local, vel
@@ -135,6 +165,7 @@ def test_translating_obstacle_through_sync_matches_oracle():
         if k % 5 == 4:
             e = rel_rms(W.densities()[fl], Wo.state()["rho"][fl])
             assert e <= 1e-5, (k, e)
+    assert np.array_equal(W.velocities()[ids], np.tile(vel.astype(np.float32), (len(ids), 1)).astype(np.float64))
     bf, bfo = W.boundary_force(), Wo.state()["boundary_force"].reshape(-1, 3)
     assert vec_rel_rms(bf[ids], bfo[ids]) <= 1e-4
     f, _ = W.reduce_rigid((c + CFG.steps * CFG.dt * vel)[None])
