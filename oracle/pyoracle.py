@@ -114,6 +114,31 @@ def lib():
         "ora_fnv1a_f64": (u64, [vp, u64]),
         "ora_fnv1a_bytes": (u64, [vp, u64]),
         "ora_dfsph_tank_hash": (u64, [u64]),
+        "ora_coupling_tank_hash": (u64, [u64, vp]),
+        "ora_world_set_visc_xsph": (None, [vp, d, d]),
+        "ora_rigid_new": (vp, [C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
+        "ora_rigid_free": (None, [vp]),
+        "ora_rigid_step": (None, [vp, d, vp, vp]),
+        "ora_rigid_pack": (None, [vp, vp]),
+        "ora_rigid_omega": (None, [vp, vp]),
+        "ora_rigid_set": (None, [vp, C.c_uint32, vp, vp, vp]),
+        "ora_inertia_solid_box": (None, [d, d, d, d, vp]),
+        "ora_inertia_solid_sphere": (None, [d, d, vp]),
+        "ora_coupler_new": (vp, [C.c_uint32]),
+        "ora_coupler_free": (None, [vp]),
+        "ora_coupler_sample_box": (None, [vp, vp, C.c_uint32, pd, d, C.c_uint32]),
+        "ora_coupler_sample_sphere": (None, [vp, vp, C.c_uint32, d, d]),
+        "ora_coupler_len": (u64, [vp]),
+        "ora_coupler_samples": (None, [vp, vp, vp, vp, vp]),
+        "ora_carve_fluid_box": (None, [vp, pd, pd, d]),
+        "ora_cw_new": (vp, [vp, u64, vp, vp, vp, vp, vp]),
+        "ora_cw_free": (None, [vp]),
+        "ora_cw_step": (None, [vp, d, u64]),
+        "ora_cw_sync": (None, [vp]),
+        "ora_cw_rigid_pack": (None, [vp, vp]),
+        "ora_cw_rigid_set": (None, [vp, C.c_uint32, vp, vp, vp]),
+        "ora_cw_wrench": (None, [vp, vp, vp]),
+        "ora_cw_time": (d, [vp]),
         "ora_mlp_forward": (None, [vp, vp, vp, u64, i32, vp]),
         "ora_mlp_grad": (None, [vp, vp, vp, vp, u64, i32, vp, pd]),
         "ora_adam": (None, [vp, vp, vp, vp, u64, d, d, d, d, C.c_uint32]),
@@ -178,6 +203,140 @@ def mass_calibrated_to_rest(rho0, dx, h, dim=3):
 def dfsph_tank_hash(steps=40):
     """particles_hash64 of the canonical DFSPH tank after `steps` steps (scenarios.rs:72-125)."""
     return lib().ora_dfsph_tank_hash(steps)
+
+
+def coupling_tank_hash(steps=80):
+    """(hash, packed rigid state [13]) of the canonical coupling scenario
+    (coupling.rs:739-767): particles_hash64 continued over State::pack."""
+    out = np.zeros(13)
+    h = lib().ora_coupling_tank_hash(steps, _p(out))
+    return h, out
+
+
+def inertia_solid_box(mass, a, b, c):
+    out = np.zeros((3, 3))
+    lib().ora_inertia_solid_box(mass, a, b, c, _p(out))
+    return out
+
+
+def inertia_solid_sphere(mass, radius):
+    out = np.zeros((3, 3))
+    lib().ora_inertia_solid_sphere(mass, radius, _p(out))
+    return out
+
+
+def _bodies(bodies):
+    """Arrays of (mass, inertia, x, q, v, omega) dicts for ora_rigid_new."""
+    nb = len(bodies)
+    get = lambda k, m: np.ascontiguousarray(np.array([b[k] for b in bodies], np.float64).reshape(nb, m))
+    return (nb, np.array([b["mass"] for b in bodies], np.float64), get("inertia", 9), get("x", 3), get("q", 4),
+            get("v", 3), get("omega", 3))
+
+
+class RigidWorld:
+    """pe-core World with UniformGravity and ExternalLoads, symplectic Euler."""
+
+    def __init__(self, bodies, gravity):
+        nb, m, i, x, q, v, w = _bodies(bodies)
+        self.n = nb
+        g = np.asarray(gravity, np.float64)
+        self.hnd = lib().ora_rigid_new(nb, _p(m), _p(i), _p(x), _p(q), _p(v), _p(w), _p(g))
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_rigid_free(self.hnd)
+
+    def step(self, dt, force=None, torque=None):
+        f = None if force is None else np.ascontiguousarray(force, np.float64).reshape(-1)
+        t = None if torque is None else np.ascontiguousarray(torque, np.float64).reshape(-1)
+        lib().ora_rigid_step(self.hnd, dt, _p(f), _p(t))
+
+    def pack(self):
+        out = np.zeros(13 * self.n)
+        lib().ora_rigid_pack(self.hnd, _p(out))
+        return out
+
+    def omega(self):
+        out = np.zeros((self.n, 3))
+        lib().ora_rigid_omega(self.hnd, _p(out))
+        return out
+
+
+class Coupler:
+    """FluidRigidCoupler sample tables over an oracle Particles."""
+
+    def __init__(self, n_bodies):
+        self.hnd = lib().ora_coupler_new(n_bodies)
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_coupler_free(self.hnd)
+
+    def sample_box(self, parts, body, half, dx, layers=1):
+        lib().ora_coupler_sample_box(self.hnd, parts.hnd, body, _pd(half), dx, layers)
+
+    def sample_sphere(self, parts, body, radius, dx):
+        lib().ora_coupler_sample_sphere(self.hnd, parts.hnd, body, radius, dx)
+
+    def samples(self):
+        n = lib().ora_coupler_len(self.hnd)
+        particle, body = np.zeros(n, np.uint64), np.zeros(n, np.uint32)
+        local, local_n = np.zeros((n, 3)), np.zeros((n, 3))
+        lib().ora_coupler_samples(self.hnd, _p(particle), _p(body), _p(local), _p(local_n))
+        return particle, body, local, local_n
+
+
+def carve_fluid_box(parts, centre, half, margin):
+    lib().ora_carve_fluid_box(parts.hnd, _pd(centre), _pd(half), margin)
+
+
+class CoupledWcsph:
+    """CoupledWcsphWorld (coupling.rs:812-866) over an oracle World: sync,
+    fluid step, reduce, ExternalLoads, one symplectic-Euler rigid step."""
+
+    def __init__(self, world, samples, bodies, gravity):
+        self.world = world  # kept alive: the coupled world points into it
+        particle, body, local, local_n = (np.ascontiguousarray(a) for a in samples)
+        particle = particle.astype(np.uint64)
+        body = body.astype(np.uint32)
+        local = local.astype(np.float64).reshape(-1)
+        local_n = local_n.astype(np.float64).reshape(-1)
+        rigid = RigidWorld(bodies, gravity)
+        self.n_bodies = rigid.n
+        self.hnd = lib().ora_cw_new(world.hnd, len(particle), _p(particle), _p(body), _p(local), _p(local_n),
+                                    rigid.hnd)
+        rigid.hnd = None  # taken over
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_cw_free(self.hnd)
+
+    def step(self, dt, nsteps=1):
+        lib().ora_cw_step(self.hnd, dt, nsteps)
+
+    def sync(self):
+        lib().ora_cw_sync(self.hnd)
+
+    def rigid_state(self):
+        """Packed (x, q, v, L) per body (world.rs:35-50)."""
+        out = np.zeros(13 * self.n_bodies)
+        lib().ora_cw_rigid_pack(self.hnd, _p(out))
+        nb = self.n_bodies
+        return dict(x=out[:3 * nb].reshape(nb, 3), q=out[3 * nb:7 * nb].reshape(nb, 4),
+                    v=out[7 * nb:10 * nb].reshape(nb, 3), l=out[10 * nb:].reshape(nb, 3))
+
+    def set_body(self, b, x=None, v=None, l=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        x, v, l = f(x), f(v), f(l)
+        lib().ora_cw_rigid_set(self.hnd, b, _p(x), _p(v), _p(l))
+
+    def wrench(self):
+        f, t = np.zeros((self.n_bodies, 3)), np.zeros((self.n_bodies, 3))
+        lib().ora_cw_wrench(self.hnd, _p(f), _p(t))
+        return f, t
+
+    def set_visc_xsph(self, visc_alpha, xsph_eps):
+        lib().ora_world_set_visc_xsph(self.world.hnd, visc_alpha, xsph_eps)
 
 
 def fnv1a_f64(vals):

@@ -813,10 +813,10 @@ struct Rng {
 // DFSPH world (crates/pe-fluid/src/dfsph.rs), fixed-dt step (dfsph.rs:704-760)
 // with the same grid (cell = support = 2h), the same unfiltered candidate
 // lists in the reference's order, the cubic spline, ψ-weighted Akinci
-// boundary masses and one-sided filter. Boundary reaction forces are not
-// restated (they never feed back into the fluid); surface tension, the
-// velocity cap and free-slip planes are the off defaults of the restated
-// scenario and are not restated either.
+// boundary masses and one-sided filter, the capped pressure correction with
+// its boundary reactions (dfsph.rs:423-528) and the free-slip planes
+// (dfsph.rs:660-711). Surface tension is the off default of both restated
+// scenarios and is not restated.
 // ---------------------------------------------------------------------------
 struct DfsphParams {
   int max_density_iters = 50, max_divergence_iters = 50;  // dfsph.rs:78-98 defaults
@@ -826,6 +826,10 @@ struct DfsphParams {
   double particle_diameter = 0.0;
   bool warm_start = true;
   double solver_omega = 1.0;
+  double max_velocity_correction = 0.0;  // 0 = unlimited
+  // free-slip planes x >= xmin, y >= ymin, x <= xmax, y <= ymax (None = off)
+  bool slip_on[4] = {false, false, false, false};
+  double slip_at[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 struct Dfsph {
@@ -834,7 +838,8 @@ struct Dfsph {
   DfsphParams prm;
   double t = 0.0, last_dt = 0.0;
   std::vector<double> alpha, kappa, kappa_v, kappa_warm, kappa_v_warm;
-  std::vector<V3> v_star, a_adv, scratch_dv;
+  std::vector<V3> v_star, a_adv, scratch_dv, dv_bound, boundary_force;
+  std::vector<std::pair<uint64_t, double>> reaction_tmp;
   std::vector<std::vector<uint64_t>> nl;
 
   void init() {  // dfsph.rs:140-180
@@ -855,6 +860,8 @@ struct Dfsph {
     v_star.assign(n, v3(0, 0, 0));
     a_adv.assign(n, v3(0, 0, 0));
     scratch_dv.assign(n, v3(0, 0, 0));
+    dv_bound.assign(n, v3(0, 0, 0));
+    boundary_force.assign(n, v3(0, 0, 0));
     nl.assign(n, {});
     rebuild_neighbours();
     compute_density();
@@ -967,17 +974,20 @@ struct Dfsph {
       if (fluid(i)) v_star[i] = add(v_star[i], scratch_dv[i]);
   }
 
-  // v* -= dt sum_j m_j (k_i / rho_i + k_j / rho_j) grad W (dfsph.rs:423-516),
-  // Jacobi: every particle's correction from the same v*.
+  // v* -= dt sum_j m_j (k_i / rho_i + k_j / rho_j) grad W (dfsph.rs:423-528),
+  // Jacobi: every particle's correction from the same v*. The part due to
+  // boundary neighbours, scaled like the capped correction, is handed to the
+  // visible boundary neighbours as -m dv_b / dt with weights min(1/r, 1e6).
   void apply_pressure_velocity_update(double dt, bool use_div) {
     const std::vector<double>& K = use_div ? kappa_v : kappa;
     const double h = P.h;
     for (auto& d : scratch_dv) d = v3(0, 0, 0);
+    for (auto& d : dv_bound) d = v3(0, 0, 0);
     for (size_t i = 0; i < P.len(); ++i) {
       if (!fluid(i)) continue;
       const V3 xi = P.x[i];
       const double rho_i = std::fmax(P.rho[i], 1e-12), ki = K[i];
-      V3 dv = v3(0, 0, 0);
+      V3 dv = v3(0, 0, 0), dv_b = v3(0, 0, 0);
       for (uint64_t j : nl[i]) {
         if (j == i) continue;
         const double mj = neighbour_mass(i, j);
@@ -986,14 +996,72 @@ struct Dfsph {
         const double kj = fluid(j) ? K[j] : 0.0;
         const double rho_j = fluid(j) ? std::fmax(P.rho[j], 1e-12) : P.rest_density;
         const double coeff = mj * (ki / rho_i + kj / rho_j);
-        dv = add(dv, mul(gw, coeff));
+        const V3 term = mul(gw, coeff);
+        dv = add(dv, term);
+        if (!fluid(j)) dv_b = add(dv_b, term);
       }
       scratch_dv[i] = mul(dv, -dt);
+      dv_bound[i] = mul(dv_b, -dt);
     }
+    const double cap = prm.max_velocity_correction, mass = P.mass;
+    const double inv_dt = 1.0 / std::fmax(dt, 1e-12);
     for (size_t i = 0; i < P.len(); ++i) {
       if (!fluid(i)) continue;
-      v_star[i] = add(v_star[i], scratch_dv[i]);
+      V3 dv = scratch_dv[i];
+      double scale = 1.0;
+      if (cap > 0.0) {
+        const double s2 = norm2(dv);
+        if (s2 > cap * cap) {
+          scale = cap / std::sqrt(s2);
+          dv = mul(dv, scale);
+        }
+      }
+      const V3 dvb = mul(dv_bound[i], scale);
+      if (norm2(dvb) > 0.0) {
+        const V3 xi = P.x[i];
+        double wsum = 0.0;
+        reaction_tmp.clear();
+        for (uint64_t j : nl[i]) {
+          if (fluid(j) || !P.boundary_sees_fluid(j, xi)) continue;
+          const double r = norm(sub(P.x[j], xi));
+          const double wt = std::fmin(1.0 / std::fmax(r, 1e-9), 1e6);
+          wsum += wt;
+          reaction_tmp.emplace_back(j, wt);
+        }
+        if (wsum > 0.0) {
+          const V3 fb_total = mul(dvb, -mass * inv_dt);
+          for (const auto& jw : reaction_tmp)
+            boundary_force[jw.first] = add(boundary_force[jw.first], mul(fb_total, jw.second / wsum));
+        }
+      }
+      v_star[i] = add(v_star[i], dv);
       if (P.dim == DIM2) v_star[i].z = 0.0;
+    }
+  }
+
+  void apply_free_slip_walls() {  // dfsph.rs:660-711
+    if (!(prm.slip_on[0] || prm.slip_on[1] || prm.slip_on[2] || prm.slip_on[3])) return;
+    for (size_t i = 0; i < P.len(); ++i) {
+      if (!fluid(i)) continue;
+      V3 x = P.x[i], v = v_star[i];
+      if (prm.slip_on[0] && x.x < prm.slip_at[0]) {
+        x.x = prm.slip_at[0];
+        if (v.x < 0.0) v.x = 0.0;
+      }
+      if (prm.slip_on[1] && x.y < prm.slip_at[1]) {
+        x.y = prm.slip_at[1];
+        if (v.y < 0.0) v.y = 0.0;
+      }
+      if (prm.slip_on[2] && x.x > prm.slip_at[2]) {
+        x.x = prm.slip_at[2];
+        if (v.x > 0.0) v.x = 0.0;
+      }
+      if (prm.slip_on[3] && x.y > prm.slip_at[3]) {
+        x.y = prm.slip_at[3];
+        if (v.y > 0.0) v.y = 0.0;
+      }
+      P.x[i] = x;
+      v_star[i] = v;
     }
   }
 
@@ -1058,10 +1126,11 @@ struct Dfsph {
     return iters;
   }
 
-  void step(double dt) {  // dfsph.rs:704-760
+  void step(double dt) {  // dfsph.rs:713-767
     dt = std::fmax(dt, 1e-12);
     last_dt = dt;
     const size_t n = P.len();
+    for (auto& f : boundary_force) f = v3(0, 0, 0);
     for (size_t i = 0; i < n; ++i) a_adv[i] = fluid(i) ? prm.gravity : v3(0, 0, 0);  // dfsph.rs:355-379
     for (size_t i = 0; i < n; ++i) {
       if (!fluid(i)) {
@@ -1078,10 +1147,12 @@ struct Dfsph {
       P.x[i] = add(P.x[i], mul(v_star[i], dt));
       if (P.dim == DIM2) P.x[i].z = 0.0;
     }
+    apply_free_slip_walls();
     rebuild_neighbours();
     compute_density();
     compute_alpha();
     correct_divergence_error(dt);
+    apply_free_slip_walls();
     for (size_t i = 0; i < n; ++i) {
       if (!fluid(i)) continue;
       P.v[i] = v_star[i];
@@ -1091,6 +1162,332 @@ struct Dfsph {
     t += dt;
   }
 };
+
+// ---------------------------------------------------------------------------
+// pe-core rigid world (crates/pe-core/src/world.rs, integrators.rs, body.rs,
+// forces.rs UniformGravity, external_loads.rs) for the coupled scenes:
+// SoA state (x, q, v, L with L the world-frame angular momentum), generators
+// in the fixed order gravity then external loads, symplectic Euler with the
+// implicit-midpoint rotation drift (4 fixed-point iterations) and the
+// quaternion renormalised after the step.
+// ---------------------------------------------------------------------------
+struct M3 {
+  double m[3][3];
+};
+struct Quat {
+  double w, x, y, z;
+};
+static M3 mat_mul(const M3& a, const M3& b) {  // body.rs:243-251
+  M3 o{};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+  return o;
+}
+static M3 mat_transpose(const M3& a) {
+  M3 o{};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[j][i];
+  return o;
+}
+static V3 mat_vec(const M3& a, V3 v) {  // body.rs:233-239
+  return v3(a.m[0][0] * v.x + a.m[0][1] * v.y + a.m[0][2] * v.z, a.m[1][0] * v.x + a.m[1][1] * v.y + a.m[1][2] * v.z,
+            a.m[2][0] * v.x + a.m[2][1] * v.y + a.m[2][2] * v.z);
+}
+static M3 invert3(const M3& a) {  // body.rs:204-229 (cofactors over the determinant)
+  const auto& m = a.m;
+  const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  const double d = 1.0 / det;
+  M3 o{};
+  o.m[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * d;
+  o.m[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * d;
+  o.m[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * d;
+  o.m[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * d;
+  o.m[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * d;
+  o.m[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * d;
+  o.m[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * d;
+  o.m[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * d;
+  o.m[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * d;
+  return o;
+}
+static M3 inertia_solid_box(double mass, double a, double b, double c) {  // body.rs:181-187
+  return M3{{{mass * (b * b + c * c) / 12.0, 0.0, 0.0},
+             {0.0, mass * (a * a + c * c) / 12.0, 0.0},
+             {0.0, 0.0, mass * (a * a + b * b) / 12.0}}};
+}
+static M3 inertia_solid_sphere(double mass, double radius) {  // body.rs:175-178
+  const double i = (2.0 / 5.0) * mass * radius * radius;
+  return M3{{{i, 0.0, 0.0}, {0.0, i, 0.0}, {0.0, 0.0, i}}};
+}
+static V3 cross(V3 a, V3 b) {  // vec3.rs:87-93
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static M3 quat_to_matrix(Quat q) {  // quat.rs:119-138
+  const double w = q.w, x = q.x, y = q.y, z = q.z;
+  return M3{{{1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)},
+             {2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x)},
+             {2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)}}};
+}
+static V3 quat_rotate(Quat q, V3 v) { return mat_vec(quat_to_matrix(q), v); }  // quat.rs:102-109
+static Quat quat_multiply(Quat a, Quat b) {  // quat.rs:81-90, Hamilton product a (x) b
+  return Quat{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+              a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+static Quat quat_normalize(Quat q) {  // quat.rs:62-77: non-finite or zero -> identity
+  if (!(std::isfinite(q.w) && std::isfinite(q.x) && std::isfinite(q.y) && std::isfinite(q.z)))
+    return Quat{1.0, 0.0, 0.0, 0.0};
+  const double n = std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  if (n <= 0.0) return Quat{1.0, 0.0, 0.0, 0.0};
+  return Quat{q.w / n, q.x / n, q.y / n, q.z / n};
+}
+// quat.rs:152-162: exp of a rotation vector, sin(th/2)/th by its series below
+// 1e-8. sin and cos are the host libm's, where the reference calls the libm
+// crate (parity unpinned in the last bit for angles above the series range).
+static Quat quat_exp_rotvec(V3 rv) {
+  const double th = norm(rv);
+  const double half = 0.5 * th;
+  const double s = th > 1e-8 ? std::sin(half) / th : 0.5 - th * th / 48.0;
+  return Quat{std::cos(half), s * rv.x, s * rv.y, s * rv.z};
+}
+
+struct RigidWorld {
+  size_t n = 0;
+  std::vector<double> mass;
+  std::vector<M3> inertia_body, inv_inertia_body;
+  std::vector<V3> x, v, l;
+  std::vector<Quat> q;
+  std::vector<V3> load_f, load_t;  // ExternalLoads buffers
+  V3 gravity{0.0, 0.0, 0.0};
+  double t = 0.0;
+
+  // World::new (world.rs:78-108) for one body described like RigidBody
+  // (position, velocity, orientation, angular velocity, body inertia).
+  size_t add_body(double m, V3 pos, V3 vel, Quat ori, V3 omega, const M3& ib) {
+    const Quat qq = quat_normalize(ori);
+    const M3 r = quat_to_matrix(qq);
+    const M3 iw = mat_mul(mat_mul(r, ib), mat_transpose(r));
+    mass.push_back(m);
+    inertia_body.push_back(ib);
+    inv_inertia_body.push_back(invert3(ib));
+    x.push_back(pos);
+    q.push_back(qq);
+    v.push_back(vel);
+    l.push_back(mat_vec(iw, omega));
+    load_f.push_back(v3(0, 0, 0));
+    load_t.push_back(v3(0, 0, 0));
+    return n++;
+  }
+  M3 inv_inertia_world(Quat qq, size_t i) const {  // world.rs:112-119
+    const M3 r = quat_to_matrix(qq);
+    return mat_mul(mat_mul(r, inv_inertia_body[i]), mat_transpose(r));
+  }
+  V3 omega_i(size_t i) const { return mat_vec(inv_inertia_world(q[i], i), l[i]); }  // world.rs:129-131
+  // World::step (world.rs:208-215) with symplectic_euler (integrators.rs:108-123)
+  // and drift_rotation (integrators.rs:75-100).
+  void step(double dt) {
+    std::vector<V3> f(n, v3(0, 0, 0)), tau(n, v3(0, 0, 0));
+    for (size_t i = 0; i < n; ++i) f[i] = add(f[i], mul(gravity, mass[i]));  // forces.rs:43-48
+    for (size_t i = 0; i < n; ++i) {                                          // external_loads.rs:150-160
+      f[i] = add(f[i], load_f[i]);
+      tau[i] = add(tau[i], load_t[i]);
+    }
+    std::vector<V3> vn(n), ln(n), xn(n);
+    std::vector<Quat> qn(n);
+    for (size_t i = 0; i < n; ++i) {
+      vn[i] = add(v[i], mul(f[i], dt / mass[i]));
+      ln[i] = add(l[i], mul(tau[i], dt));
+    }
+    for (size_t i = 0; i < n; ++i) {
+      const Quat qi = q[i];
+      V3 w = mat_vec(inv_inertia_world(qi, i), ln[i]);
+      for (int it = 0; it < 4; ++it) {
+        const Quat q_mid = quat_multiply(quat_exp_rotvec(mul(w, 0.5 * dt)), qi);
+        w = mat_vec(inv_inertia_world(q_mid, i), ln[i]);
+      }
+      qn[i] = quat_normalize(quat_multiply(quat_exp_rotvec(mul(w, dt)), qi));
+    }
+    for (size_t i = 0; i < n; ++i) xn[i] = add(x[i], mul(vn[i], dt));
+    x = xn;
+    v = vn;
+    l = ln;
+    for (size_t i = 0; i < n; ++i) q[i] = quat_normalize(qn[i]);
+    t += dt;
+  }
+  // State::pack (world.rs:35-50): x..., q (w, x, y, z)..., v..., L...
+  std::vector<double> pack() const {
+    std::vector<double> o;
+    for (auto& a : x) o.insert(o.end(), {a.x, a.y, a.z});
+    for (auto& a : q) o.insert(o.end(), {a.w, a.x, a.y, a.z});
+    for (auto& a : v) o.insert(o.end(), {a.x, a.y, a.z});
+    for (auto& a : l) o.insert(o.end(), {a.x, a.y, a.z});
+    return o;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FluidRigidCoupler (crates/pe-fluid/src/coupling.rs:70-349).
+// ---------------------------------------------------------------------------
+struct RigidSample {
+  size_t body;
+  V3 local, local_n;
+  size_t particle;
+};
+static const double RIGID_PSI_SCALE = 1.0;  // coupling.rs:98-103
+
+struct Coupler {
+  std::vector<RigidSample> samples;
+  std::vector<V3> force, torque;
+
+  void resize_bodies(size_t nb) {
+    force.resize(nb, v3(0, 0, 0));
+    torque.resize(nb, v3(0, 0, 0));
+  }
+  void push_sample(Particles& P, size_t body, V3 local, double psi, V3 n_out) {  // coupling.rs:267-286
+    const size_t particle = P.len();
+    P.push_boundary_oriented(local, psi, n_out);
+    samples.push_back(RigidSample{body, local, n_out, particle});
+    if (body >= force.size()) resize_bodies(body + 1);
+  }
+  // coupling.rs:104-230: an axis-aligned box surface, samples at body-frame
+  // positions (the caller places them), 2-D faces unfiltered (n_out = 0).
+  void sample_box(Particles& P, size_t body, V3 half, double dx, size_t layers) {
+    const double psi = (P.dim == DIM2 ? dx * dx : dx * dx * dx) * RIGID_PSI_SCALE;
+    layers = std::max<size_t>(layers, 1);
+    const size_t nx = std::max<size_t>((size_t)std::round(2.0 * half.x / dx), 1);
+    const size_t ny = std::max<size_t>((size_t)std::round(2.0 * half.y / dx), 1);
+    const V3 z0 = v3(0, 0, 0);
+    for (size_t layer = 0; layer < layers; ++layer) {
+      const double off = (double)layer * dx;
+      if (P.dim == DIM2) {
+        for (size_t i = 0; i <= nx; ++i) {
+          const double x = -half.x + (double)i * (2.0 * half.x) / (double)nx;
+          push_sample(P, body, v3(x, -half.y - off, 0.0), psi, z0);
+          push_sample(P, body, v3(x, half.y + off, 0.0), psi, z0);
+        }
+        for (size_t j = 1; j < ny; ++j) {
+          const double y = -half.y + (double)j * (2.0 * half.y) / (double)ny;
+          push_sample(P, body, v3(-half.x - off, y, 0.0), psi, z0);
+          push_sample(P, body, v3(half.x + off, y, 0.0), psi, z0);
+        }
+        continue;
+      }
+      const size_t nz = std::max<size_t>((size_t)std::round(2.0 * half.z / dx), 1);
+      for (size_t i = 0; i <= nx; ++i)
+        for (size_t j = 0; j <= ny; ++j) {
+          const double x = -half.x + (double)i * (2.0 * half.x) / (double)nx;
+          const double y = -half.y + (double)j * (2.0 * half.y) / (double)ny;
+          push_sample(P, body, v3(x, y, -half.z - off), psi, v3(0.0, 0.0, -1.0));
+          push_sample(P, body, v3(x, y, half.z + off), psi, v3(0.0, 0.0, 1.0));
+        }
+      for (size_t i = 0; i <= nx; ++i)
+        for (size_t k = 1; k < nz; ++k) {
+          const double x = -half.x + (double)i * (2.0 * half.x) / (double)nx;
+          const double z = -half.z + (double)k * (2.0 * half.z) / (double)nz;
+          push_sample(P, body, v3(x, -half.y - off, z), psi, v3(0.0, -1.0, 0.0));
+          push_sample(P, body, v3(x, half.y + off, z), psi, v3(0.0, 1.0, 0.0));
+        }
+      for (size_t j = 1; j < ny; ++j)
+        for (size_t k = 1; k < nz; ++k) {
+          const double y = -half.y + (double)j * (2.0 * half.y) / (double)ny;
+          const double z = -half.z + (double)k * (2.0 * half.z) / (double)nz;
+          push_sample(P, body, v3(-half.x - off, y, z), psi, v3(-1.0, 0.0, 0.0));
+          push_sample(P, body, v3(half.x + off, y, z), psi, v3(1.0, 0.0, 0.0));
+        }
+    }
+  }
+  // coupling.rs:233-264: circle in 2-D, golden spiral in 3-D, radial normals.
+  void sample_sphere(Particles& P, size_t body, double radius, double dx) {
+    const double psi = (P.dim == DIM2 ? dx * dx : dx * dx * dx) * RIGID_PSI_SCALE;
+    if (P.dim == DIM2) {
+      const size_t n = std::max<size_t>((size_t)std::round(2.0 * PI * radius / dx), 8);
+      for (size_t i = 0; i < n; ++i) {
+        const double th = 2.0 * PI * (double)i / (double)n;
+        push_sample(P, body, v3(radius * std::cos(th), radius * std::sin(th), 0.0), psi,
+                    v3(std::cos(th), std::sin(th), 0.0));
+      }
+      return;
+    }
+    const double area = 4.0 * PI * radius * radius;
+    const size_t n = (size_t)std::fmax(std::round(area / (dx * dx)), 12.0);
+    const double ga = PI * (3.0 - std::sqrt(5.0));
+    for (size_t i = 0; i < n; ++i) {
+      const double y = 1.0 - 2.0 * ((double)i + 0.5) / (double)n;
+      const double r_xy = std::sqrt(std::fmax(1.0 - y * y, 0.0));
+      const double th = ga * (double)i;
+      push_sample(P, body, v3(radius * r_xy * std::cos(th), radius * y, radius * r_xy * std::sin(th)), psi,
+                  v3(r_xy * std::cos(th), y, r_xy * std::sin(th)));
+    }
+  }
+  // coupling.rs:289-305
+  void sync_from_rigid(Particles& P, const RigidWorld& R) const {
+    for (const auto& s : samples) {
+      const V3 x = R.x[s.body];
+      const Quat q = R.q[s.body];
+      const V3 v = R.v[s.body];
+      const V3 w = R.omega_i(s.body);
+      const V3 r = quat_rotate(q, s.local);
+      P.x[s.particle] = add(x, r);
+      P.v[s.particle] = add(v, cross(w, r));
+      P.n_out[s.particle] = quat_rotate(q, s.local_n);
+      if (P.dim == DIM2) {
+        P.x[s.particle].z = 0.0;
+        P.v[s.particle].z = 0.0;
+        P.n_out[s.particle].z = 0.0;
+      }
+    }
+  }
+  // coupling.rs:308-330, com = the rigid state's positions
+  void reduce_with_state(const Particles& P, const std::vector<V3>& bf, const RigidWorld& R) {
+    for (auto& f : force) f = v3(0, 0, 0);
+    for (auto& t : torque) t = v3(0, 0, 0);
+    for (const auto& s : samples) {
+      const V3 fb = s.particle < bf.size() ? bf[s.particle] : v3(0, 0, 0);
+      force[s.body] = add(force[s.body], fb);
+      const V3 r = sub(P.x[s.particle], R.x[s.body]);
+      torque[s.body] = add(torque[s.body], cross(r, fb));
+    }
+  }
+  // ExternalLoads::set_all then clear around the rigid step (coupling.rs:393-408, 853-865)
+  void load_into(RigidWorld& R) const {
+    for (size_t b = 0; b < R.n; ++b) {
+      R.load_f[b] = b < force.size() ? force[b] : v3(0, 0, 0);
+      R.load_t[b] = b < torque.size() ? torque[b] : v3(0, 0, 0);
+    }
+  }
+};
+
+// carve_fluid_box (coupling.rs:561-600): drop fluid particles strictly inside
+// the box grown by margin; every other particle keeps its order.
+static void carve_fluid_box(Particles& P, V3 c, V3 half, double margin) {
+  const double hx = half.x + margin, hy = half.y + margin, hz = half.z + margin;
+  Particles Q;
+  Q.dim = P.dim;
+  Q.rest_density = P.rest_density;
+  Q.mass = P.mass;
+  Q.h = P.h;
+  for (size_t i = 0; i < P.len(); ++i) {
+    if (P.kind[i] == KIND_FLUID) {
+      const V3 d = sub(P.x[i], c);
+      const bool inside = P.dim == DIM2 ? (std::fabs(d.x) < hx && std::fabs(d.y) < hy)
+                                        : (std::fabs(d.x) < hx && std::fabs(d.y) < hy && std::fabs(d.z) < hz);
+      if (inside) continue;
+    }
+    Q.x.push_back(P.x[i]);
+    Q.v.push_back(P.v[i]);
+    Q.a.push_back(P.a[i]);
+    Q.rho.push_back(P.rho[i]);
+    Q.p.push_back(P.p[i]);
+    Q.kind.push_back(P.kind[i]);
+    Q.psi.push_back(P.psi[i]);
+    Q.n_out.push_back(P.n_out[i]);
+  }
+  P = Q;
+}
+
+static double archimedes_box_draft(double rho_body, double rho_fluid, double height) {  // coupling.rs:410-414
+  return (rho_body / std::fmax(rho_fluid, 1e-12)) * height;
+}
 
 // FNV-1a over little-endian f64 bits of fluid (x, v) (diagnostics.rs:7-33).
 static uint64_t particles_hash64(const Particles& P) {
@@ -1113,6 +1510,92 @@ static uint64_t particles_hash64(const Particles& P) {
     eat(P.v[i].z);
   }
   return hsh;
+}
+
+// CoupledDfsphWorld::step (coupling.rs:393-408) and CoupledWcsphWorld::step
+// (coupling.rs:853-865): samples from the rigid pose, the fluid step, the
+// wrench reduction, ExternalLoads around one symplectic-Euler rigid step.
+struct CoupledDfsph {
+  Dfsph D;
+  RigidWorld R;
+  Coupler C;
+  void step(double dt) {
+    C.sync_from_rigid(D.P, R);
+    D.rebuild_neighbours();
+    D.compute_density();
+    D.compute_alpha();
+    D.step(dt);
+    C.reduce_with_state(D.P, D.boundary_force, R);
+    C.load_into(R);
+    R.step(dt);
+    for (size_t b = 0; b < R.n; ++b) R.load_f[b] = R.load_t[b] = v3(0, 0, 0);
+  }
+};
+
+struct CoupledWcsph {
+  World* W = nullptr;  // owned by the caller
+  RigidWorld R;
+  Coupler C;
+  void step(double dt) {
+    C.sync_from_rigid(W->P, R);
+    W->step(dt);
+    C.reduce_with_state(W->P, W->boundary_force, R);
+    C.load_into(R);
+    R.step(dt);
+    for (size_t b = 0; b < R.n; ++b) R.load_f[b] = R.load_t[b] = v3(0, 0, 0);
+  }
+};
+
+// make_floating_box_2d (coupling.rs:603-686): DFSPH tank of nx x ny fluid
+// particles with a two-layer Akinci shell, a box body of density rho_body
+// floating at its Archimedes draft (+dx/2), the fluid inside it carved out.
+static void make_floating_box_2d(CoupledDfsph& CW, double dx, double tank_w, double water_h, double hx, double hy,
+                                 double rho_body) {
+  const double rho0 = 1000.0, hsm = 2.2 * dx, g = 9.80665;
+  Particles& P = CW.D.P;
+  P.dim = DIM2;
+  P.rest_density = rho0;
+  P.mass = mass_calibrated_to_rest(rho0, dx, hsm, DIM2);
+  P.h = hsm;
+  const size_t nx = std::max<size_t>((size_t)std::floor(tank_w / dx), 4);
+  const size_t ny = std::max<size_t>((size_t)std::floor(water_h / dx), 4);
+  fill_box(P, v3(0.5 * dx, 0.5 * dx, 0.0), nx, ny, 1, dx, v3(0, 0, 0));
+  fill_box_boundary(P, v3(0.0, 0.0, 0.0), v3((double)nx * dx, water_h * 2.5, 0.0), dx, 2);
+  recomputed_akinci_psi(P, true);
+  const double draft = archimedes_box_draft(rho_body, rho0, 2.0 * hy);
+  const double y_com = (water_h - draft) + hy + 0.5 * dx;
+  const double x_com = 0.5 * tank_w;
+  const V3 half = v3(hx, hy, 0.0);
+  carve_fluid_box(P, v3(x_com, y_com, 0.0), half, 1.25 * dx);
+  CW.C.resize_bodies(1);
+  CW.C.sample_box(P, 0, half, dx, 1);
+  const double mass_b = rho_body * (4.0 * hx * hy);
+  CW.R.gravity = v3(0.0, -g, 0.0);
+  CW.R.add_body(mass_b, v3(x_com, y_com, 0.0), v3(0, 0, 0), Quat{1.0, 0.0, 0.0, 0.0}, v3(0, 0, 0),
+                inertia_solid_box(mass_b, 2.0 * hx, 2.0 * hy, 1.0));
+  DfsphParams& prm = CW.D.prm;
+  prm.gravity = v3(0.0, -g, 0.0);
+  prm.particle_diameter = dx;
+  prm.density_error_tol = 0.01;
+  prm.divergence_error_tol = 0.01;
+  prm.max_density_iters = 40;
+  prm.max_divergence_iters = 25;
+  prm.xsph_eps = 0.15;
+  prm.warm_start = false;
+  prm.max_velocity_correction = 0.8;
+  prm.slip_on[0] = prm.slip_on[1] = prm.slip_on[2] = true;
+  prm.slip_at[0] = 0.0;
+  prm.slip_at[1] = 0.0;
+  prm.slip_at[2] = (double)nx * dx;
+  // the body at rest: samples at x + R local, velocity v + w x r, z zeroed
+  for (const auto& smp : CW.C.samples) {
+    const V3 r = quat_rotate(CW.R.q[0], smp.local);
+    P.x[smp.particle] = add(CW.R.x[0], r);
+    P.v[smp.particle] = add(v3(0, 0, 0), cross(v3(0, 0, 0), r));
+    P.x[smp.particle].z = 0.0;
+    P.v[smp.particle].z = 0.0;
+  }
+  CW.D.init();
 }
 
 }  // namespace ora
@@ -1151,6 +1634,31 @@ uint64_t ora_dfsph_tank_hash(uint64_t steps) {
   D.init();
   for (uint64_t k = 0; k < steps; ++k) D.step(1.0 / 120.0);
   return particles_hash64(D.P);
+}
+
+// The canonical coupling scenario (coupling.rs:739-767): make_floating_box_2d
+// (dx 0.05, tank 0.8, water 0.35, half extents 0.12 x 0.08, body density
+// 500), `steps` coupled steps of dt = 1/240; particles_hash64 of the fluid
+// continued over the rigid state's packed f64 bits (the reference locks
+// 0xe9a6a84fef4ce9d5 for 80 steps). rigid_out, when non-null, receives the
+// 13 packed doubles.
+uint64_t ora_coupling_tank_hash(uint64_t steps, double* rigid_out) {
+  using namespace ora;
+  CoupledDfsph CW;
+  make_floating_box_2d(CW, 0.05, 0.8, 0.35, 0.12, 0.08, 500.0);
+  for (uint64_t k = 0; k < steps; ++k) CW.step(1.0 / 240.0);
+  uint64_t hsh = particles_hash64(CW.D.P);
+  const std::vector<double> pk = CW.R.pack();
+  for (double d : pk) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    for (int k = 0; k < 8; ++k) {
+      hsh ^= (b >> (8 * k)) & 0xffULL;
+      hsh *= 0x100000001b3ULL;
+    }
+  }
+  if (rigid_out) std::memcpy(rigid_out, pk.data(), pk.size() * 8);
+  return hsh;
 }
 
 typedef struct {
@@ -1292,6 +1800,134 @@ void* ora_world_new(const OraParams* prm, uint64_t n, const double* x, const dou
   return W;
 }
 void ora_world_free(void* w) { delete (World*)w; }
+void ora_world_set_visc_xsph(void* w, double visc_alpha, double xsph_eps) {
+  ((World*)w)->params.visc.alpha = visc_alpha;
+  ((World*)w)->params.xsph_eps = xsph_eps;
+}
+
+// --- rigid world (pe-core) ---------------------------------------------------
+// Bodies: mass, body inertia (9, row-major), position, orientation (w, x, y,
+// z), velocity, angular velocity; gravity g.
+void* ora_rigid_new(uint32_t nb, const double* mass, const double* inertia, const double* x, const double* q,
+                    const double* v, const double* omega, const double* g) {
+  RigidWorld* R = new RigidWorld();
+  R->gravity = v3(g[0], g[1], g[2]);
+  for (uint32_t b = 0; b < nb; ++b) {
+    M3 ib{};
+    for (int i = 0; i < 9; ++i) ib.m[i / 3][i % 3] = inertia[9 * b + i];
+    R->add_body(mass[b], v3(x[3 * b], x[3 * b + 1], x[3 * b + 2]),
+                v3(v[3 * b], v[3 * b + 1], v[3 * b + 2]), Quat{q[4 * b], q[4 * b + 1], q[4 * b + 2], q[4 * b + 3]},
+                v3(omega[3 * b], omega[3 * b + 1], omega[3 * b + 2]), ib);
+  }
+  return R;
+}
+void ora_rigid_free(void* r) { delete (RigidWorld*)r; }
+void ora_inertia_solid_box(double mass, double a, double b, double c, double* out) {
+  std::memcpy(out, inertia_solid_box(mass, a, b, c).m, 72);
+}
+void ora_inertia_solid_sphere(double mass, double radius, double* out) {
+  std::memcpy(out, inertia_solid_sphere(mass, radius).m, 72);
+}
+// One World::step with ExternalLoads (force, torque per body) set around it.
+void ora_rigid_step(void* r, double dt, const double* force, const double* torque) {
+  RigidWorld* R = (RigidWorld*)r;
+  for (size_t b = 0; b < R->n; ++b) {
+    R->load_f[b] = force ? v3(force[3 * b], force[3 * b + 1], force[3 * b + 2]) : v3(0, 0, 0);
+    R->load_t[b] = torque ? v3(torque[3 * b], torque[3 * b + 1], torque[3 * b + 2]) : v3(0, 0, 0);
+  }
+  R->step(dt);
+  for (size_t b = 0; b < R->n; ++b) R->load_f[b] = R->load_t[b] = v3(0, 0, 0);
+}
+void ora_rigid_pack(void* r, double* out) {
+  const std::vector<double> pk = ((RigidWorld*)r)->pack();
+  std::memcpy(out, pk.data(), pk.size() * 8);
+}
+void ora_rigid_omega(void* r, double* out) {
+  RigidWorld* R = (RigidWorld*)r;
+  for (size_t b = 0; b < R->n; ++b) {
+    const V3 w = R->omega_i(b);
+    out[3 * b] = w.x;
+    out[3 * b + 1] = w.y;
+    out[3 * b + 2] = w.z;
+  }
+}
+// Overwrite body b's position, velocity and angular momentum (tests kick a body).
+void ora_rigid_set(void* r, uint32_t b, const double* x, const double* v, const double* l) {
+  RigidWorld* R = (RigidWorld*)r;
+  if (x) R->x[b] = v3(x[0], x[1], x[2]);
+  if (v) R->v[b] = v3(v[0], v[1], v[2]);
+  if (l) R->l[b] = v3(l[0], l[1], l[2]);
+}
+
+// --- coupler (coupling.rs:70-349) over a caller-owned particle set ------------
+// sample_box / sample_sphere append samples to an oracle particle set.
+void* ora_coupler_new(uint32_t nb) {
+  Coupler* c = new Coupler();
+  c->resize_bodies(nb);
+  return c;
+}
+void ora_coupler_free(void* c) { delete (Coupler*)c; }
+void ora_coupler_sample_box(void* c, void* parts, uint32_t body, const double* half, double dx, uint32_t layers) {
+  ((Coupler*)c)->sample_box(*(Particles*)parts, body, v3(half[0], half[1], half[2]), dx, layers);
+}
+void ora_coupler_sample_sphere(void* c, void* parts, uint32_t body, double radius, double dx) {
+  ((Coupler*)c)->sample_sphere(*(Particles*)parts, body, radius, dx);
+}
+uint64_t ora_coupler_len(void* c) { return ((Coupler*)c)->samples.size(); }
+// samples as particle index, body, local (3), local_n (3)
+void ora_coupler_samples(void* c, uint64_t* particle, uint32_t* body, double* local, double* local_n) {
+  const auto& S = ((Coupler*)c)->samples;
+  for (size_t k = 0; k < S.size(); ++k) {
+    particle[k] = S[k].particle;
+    body[k] = (uint32_t)S[k].body;
+    const double a[6] = {S[k].local.x, S[k].local.y, S[k].local.z, S[k].local_n.x, S[k].local_n.y, S[k].local_n.z};
+    std::memcpy(local + 3 * k, a, 24);
+    std::memcpy(local_n + 3 * k, a + 3, 24);
+  }
+}
+void ora_carve_fluid_box(void* parts, const double* c, const double* half, double margin) {
+  carve_fluid_box(*(Particles*)parts, v3(c[0], c[1], c[2]), v3(half[0], half[1], half[2]), margin);
+}
+
+// --- coupled WCSPH world: an oracle World (caller-owned), the coupler's
+// samples (as ora_coupler_samples gives them) and a rigid world (taken over).
+void* ora_cw_new(void* world, uint64_t ns, const uint64_t* particle, const uint32_t* body, const double* local,
+                 const double* local_n, void* rigid) {
+  CoupledWcsph* cw = new CoupledWcsph();
+  cw->W = (World*)world;
+  RigidWorld* R = (RigidWorld*)rigid;
+  cw->R = *R;
+  delete R;
+  cw->C.resize_bodies(cw->R.n);
+  for (uint64_t k = 0; k < ns; ++k)
+    cw->C.samples.push_back(RigidSample{body[k], v3(local[3 * k], local[3 * k + 1], local[3 * k + 2]),
+                                        v3(local_n[3 * k], local_n[3 * k + 1], local_n[3 * k + 2]), particle[k]});
+  return cw;
+}
+void ora_cw_free(void* cw) { delete (CoupledWcsph*)cw; }
+void ora_cw_step(void* cw, double dt, uint64_t steps) {
+  for (uint64_t k = 0; k < steps; ++k) ((CoupledWcsph*)cw)->step(dt);
+}
+void ora_cw_sync(void* cw) {
+  CoupledWcsph* c = (CoupledWcsph*)cw;
+  c->C.sync_from_rigid(c->W->P, c->R);
+}
+void ora_cw_rigid_pack(void* cw, double* out) {
+  const std::vector<double> pk = ((CoupledWcsph*)cw)->R.pack();
+  std::memcpy(out, pk.data(), pk.size() * 8);
+}
+void ora_cw_rigid_set(void* cw, uint32_t b, const double* x, const double* v, const double* l) {
+  ora_rigid_set(&((CoupledWcsph*)cw)->R, b, x, v, l);
+}
+void ora_cw_wrench(void* cw, double* force, double* torque) {
+  const Coupler& C = ((CoupledWcsph*)cw)->C;
+  for (size_t b = 0; b < C.force.size(); ++b) {
+    const double a[6] = {C.force[b].x, C.force[b].y, C.force[b].z, C.torque[b].x, C.torque[b].y, C.torque[b].z};
+    std::memcpy(force + 3 * b, a, 24);
+    std::memcpy(torque + 3 * b, a + 3, 24);
+  }
+}
+double ora_cw_time(void* cw) { return ((CoupledWcsph*)cw)->R.t; }
 void ora_world_set_threads(void* w, int32_t threads) { ((World*)w)->threads = threads < 1 ? 1 : threads; }
 void ora_world_step(void* w, double dt, uint64_t nsteps) {
   for (uint64_t s = 0; s < nsteps; ++s) ((World*)w)->step(dt);

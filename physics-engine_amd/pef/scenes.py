@@ -193,53 +193,14 @@ def sample_box(parts, centre, half, dx: float, layers: int = 1):
     `centre`: one-sided Akinci samples of an axis-aligned box surface with
     outward face normals and psi = dx^dim (RIGID_PSI_SCALE = 1), appended in
     the reference's loop order (z faces, then y faces, then x faces per layer;
-    2-D: y faces, then x faces)."""
+    2-D: y faces, then x faces) and placed at centre + local."""
     import pef
+    from pef.coupling import box_samples
 
-    cx, cy, cz = (float(v) for v in centre)
-    hx, hy, hz = (float(v) for v in half)
+    xs, ns = box_samples(parts.dim, half, dx, layers)
     psi = dx * dx if parts.dim == pef.D2 else dx * dx * dx
-    layers = max(layers, 1)
-    xs, ns = [], []
-
-    def push(p, n):
-        xs.append((cx + p[0], cy + p[1], cz + p[2]))
-        ns.append(n)
-
-    nx = max(_round_half_away(2.0 * hx / dx), 1)
-    ny = max(_round_half_away(2.0 * hy / dx), 1)
-    for layer in range(layers):
-        off = layer * dx
-        if parts.dim == pef.D2:
-            for i in range(nx + 1):
-                x = -hx + i * (2.0 * hx) / nx
-                push((x, -hy - off, 0.0), (0.0, 0.0, 0.0))
-                push((x, hy + off, 0.0), (0.0, 0.0, 0.0))
-            for j in range(1, ny):
-                y = -hy + j * (2.0 * hy) / ny
-                push((-hx - off, y, 0.0), (0.0, 0.0, 0.0))
-                push((hx + off, y, 0.0), (0.0, 0.0, 0.0))
-            continue
-        nz = max(_round_half_away(2.0 * hz / dx), 1)
-        for i in range(nx + 1):
-            for j in range(ny + 1):
-                x = -hx + i * (2.0 * hx) / nx
-                y = -hy + j * (2.0 * hy) / ny
-                push((x, y, -hz - off), (0.0, 0.0, -1.0))
-                push((x, y, hz + off), (0.0, 0.0, 1.0))
-        for i in range(nx + 1):
-            for k in range(1, nz):
-                x = -hx + i * (2.0 * hx) / nx
-                z = -hz + k * (2.0 * hz) / nz
-                push((x, -hy - off, z), (0.0, -1.0, 0.0))
-                push((x, hy + off, z), (0.0, 1.0, 0.0))
-        for j in range(1, ny):
-            for k in range(1, nz):
-                y = -hy + j * (2.0 * hy) / ny
-                z = -hz + k * (2.0 * hz) / nz
-                push((-hx - off, y, z), (-1.0, 0.0, 0.0))
-                push((hx + off, y, z), (1.0, 0.0, 0.0))
     n = len(xs)
-    parts._append(np.asarray(xs, float), np.zeros((n, 3)), np.full(n, pef.BOUNDARY, np.uint8), np.full(n, psi),
-                  np.asarray(ns, float))
+    x = np.asarray(centre, float) + np.asarray(xs, float).reshape(-1, 3)
+    parts._append(x, np.zeros((n, 3)), np.full(n, pef.BOUNDARY, np.uint8), np.full(n, psi),
+                  np.asarray(ns, float).reshape(-1, 3))
     return n

@@ -1,0 +1,236 @@
+"""CoupledWcsphWorld on the device: the reference's coupling oracles
+(tests/coupling_oracles.rs) with their locked bars, and the coupled run
+against the f64 oracle's coupled world.
+
+Bars:
+  - the floating-box scene built through the device scene setup is the
+    oracle's scene bit for bit (positions, kinds, psi, normals);
+  - over 400 coupled steps (the start kicks the box up 0.4 m and it falls
+    back) the body follows the oracle's body: every 20 steps
+    |y_gpu - y_oracle| <= 1e-4 dx and |v_gpu - v_oracle| <= 5e-4 of the
+    oracle's speed, fluid density RMS relative <= 1e-5. Measured: 8e-7 m,
+    4.5e-5 and 3.8e-6; the oracle itself moves 5.5e-7 m, 4e-5 and 4.5e-6 when
+    its fluid positions are perturbed by 3e-8 m, FP32's resolution here;
+  - Archimedes draft after 2 000 steps of 2.5e-4 s: rel < 0.10 and the
+    reference draft within 2 % of the analytic one (coupling_oracles.rs:29-52);
+  - heave period after release: >= 3 up-crossings and rel < 0.15 against
+    2 pi sqrt((m + m_a) / (rho g A)) (coupling_oracles.rs:54-110);
+  - closed coupled momentum without walls or gravity: drift < 1e-3 over 50
+    steps (coupling_oracles.rs:112-165);
+  - no penetration of the floating box: < 2 dx over 800 steps
+    (coupling_oracles.rs:167-186);
+  - a released submerged sphere's initial acceleration within 10 % of
+    g (rho_s - rho_f) / (rho_s + rho_f / 2) (coupling_oracles.rs:188-296).
+    The device gives 3.948 m/s^2 (rel 0.094) and so does the f64 oracle on
+    the same scene, to four digits; the reference's notes record rel 0.0039
+    for this setup (docs/FLUID.md:240), which neither reproduces.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pef
+import pyoracle as O
+from helpers import rel_rms
+from pef import coupling as CP
+
+pytestmark = pytest.mark.gpu
+
+G = CP.STANDARD_GRAVITY
+DX, TANK_W, WATER_H, HX, HY, RHO_B = 0.05, 0.9, 0.40, 0.12, 0.08, 500.0
+
+
+def _oracle_floating_box(dx, tank_w, water_h, hx, hy, rho_b):
+    """make_floating_box_wcsph_2d (coupling.rs:869-934) built from the oracle."""
+    rho0, hsm = 1000.0, 2.2 * dx
+    m = O.mass_calibrated_to_rest(rho0, dx, hsm, 2)
+    P = O.Particles(2, rho0, m, hsm)
+    nx, ny = max(int(math.floor(tank_w / dx)), 4), max(int(math.floor(water_h / dx)), 4)
+    P.fill_box((0.5 * dx, 0.5 * dx, 0.0), nx, ny, 1, dx)
+    P.fill_box_boundary((0.0, 0.0, 0.0), (nx * dx, water_h * 2.5, 0.0), dx, 2)
+    P.recompute_psi()
+    y_com = (water_h - CP.archimedes_box_draft(rho_b, rho0, 2.0 * hy)) + hy + 0.25 * dx
+    x_com = 0.5 * tank_w
+    O.carve_fluid_box(P, (x_com, y_com, 0.0), (hx, hy, 0.0), dx)
+    C = O.Coupler(1)
+    C.sample_box(P, 0, (hx, hy, 0.0), dx, 1)
+    a = P.arrays()
+    particle, body, local, local_n = C.samples()
+    for k, i in enumerate(particle):
+        p = CP.quat_rotate((1.0, 0.0, 0.0, 0.0), tuple(local[k]))
+        a["x"][i] = (x_com + p[0], y_com + p[1], 0.0)
+        a["v"][i] = 0.0
+    mass_b = rho_b * (4.0 * hx * hy)
+    bodies = [dict(mass=mass_b, inertia=O.inertia_solid_box(mass_b, 2 * hx, 2 * hy, 1.0).reshape(9),
+                   x=(x_com, y_com, 0.0), q=(1.0, 0.0, 0.0, 0.0), v=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0))]
+    prm = dict(dim=2, rest_density=rho0, mass=m, h=hsm, sound_speed=40.0, visc_alpha=0.15, xsph_eps=0.1,
+               gravity=(0.0, -G, 0.0), adapt_sound_speed=False)
+    W = O.World(prm, a["x"], a["kind"], v=a["v"], psi=a["psi"], n_out=a["n_out"], threads=4)
+    return O.CoupledWcsph(W, (particle, body, local, local_n), bodies, (0.0, -G, 0.0)), a
+
+
+def test_floating_box_scene_is_the_oracle_scene():
+    w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    _, a = _oracle_floating_box(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    P = w.particles
+    assert len(P) == len(a["x"])
+    for k in ("x", "kind", "psi", "n_out"):
+        assert np.array_equal(getattr(P, k), a[k]), k
+
+
+def test_coupled_floating_box_follows_the_oracle():
+    w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    cw, a = _oracle_floating_box(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    fl = a["kind"] == 0
+    dt = 2.5e-4
+    for k in range(20):
+        for _ in range(20):
+            w.step(dt)
+        cw.step(dt, 20)
+        so = cw.rigid_state()
+        dy = abs(w.rigid.x[0][1] - so["x"][0, 1])
+        vmax = max(np.abs(so["v"][0]).max(), 1e-3)
+        dv = np.abs(np.asarray(w.rigid.v[0]) - so["v"][0]).max() / vmax
+        e = rel_rms(w.fluid.densities()[fl], cw.world.state()["rho"][fl])
+        print(f"step {20 * (k + 1)}: |dy| {dy:.3e} m, dv {dv:.3e}, density {e:.3e}", flush=True)
+        assert dy <= 1e-4 * DX and dv <= 5e-4 and e <= 1e-5, (k, dy, dv, e)
+    fo, _ = cw.wrench()
+    assert np.linalg.norm(w.force[0] - fo[0]) <= 1e-2 * np.linalg.norm(fo[0])
+
+
+def test_floating_box_archimedes_draft():
+    w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    for _ in range(2000):
+        w.step(2.5e-4)
+    d_sim, d_ref, rel = CP.floating_box_draft_error_wcsph(w, HY, WATER_H)
+    d_an = CP.archimedes_box_draft(RHO_B, 1000.0, 2.0 * HY)
+    print(f"draft_sim={d_sim:.5f} draft_ref={d_ref:.5f} analytic={d_an:.5f} rel={rel:.4f}")
+    assert rel < 0.10
+    assert abs(d_ref - d_an) / d_an < 0.02
+
+
+def test_floating_box_heave_period():
+    w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    dt, kick = 2.5e-4, 0.04
+    for _ in range(2000):
+        w.step(dt)
+    y_eq = w.rigid.x[0][1]
+    p = w.fluid.get_params()
+    p.visc_alpha, p.xsph_epsilon = 0.01, 0.0
+    w.fluid.set_params(p)
+    x = w.rigid.x[0]
+    w.rigid.x[0] = (x[0], y_eq - kick, x[2])
+    w.rigid.v[0] = (0.0, 0.0, 0.0)
+    samples = []
+    for _ in range(16000):
+        w.step(dt)
+        samples.append((w.rigid.t, w.rigid.x[0][1]))
+    t0 = samples[0][0]
+    win = [(t, y) for t, y in samples if t0 + 0.1 <= t <= t0 + 4.0]
+    mass, a_wp = w.rigid.mass[0], 2.0 * HX
+    t_ref = CP.heave_period_with_added_mass(mass, CP.barge_heave_added_mass_2d(1000.0, a_wp), 1000.0, a_wp, G)
+    t_sim = CP.estimate_heave_period(win)
+    mean = sum(y for _, y in win) / len(win)
+    ncross = sum(1 for i in range(1, len(win)) if win[i - 1][1] - mean <= 0.0 < win[i][1] - mean)
+    rel = abs(t_sim - t_ref) / t_ref
+    print(f"heave T_sim={t_sim:.4f} T_ref={t_ref:.4f} ncross={ncross} rel={rel:.4f}")
+    assert ncross >= 3
+    assert rel < 0.15
+
+
+def test_coupled_momentum_near_conserved_no_walls():
+    dx = 0.05
+    h = 2.2 * dx
+    m = pef.mass_calibrated_to_rest(1000.0, dx, h, pef.D2)
+    parts = pef.Particles(pef.D2, 1000.0, m, h)
+    pef.fill_box(parts, (0.0, 0.0, 0.0), 6, 6, 1, dx, (0.2, 0.0, 0.0))
+    coupler = CP.FluidRigidCoupler(1)
+    coupler.sample_box(parts, 0, (0.08, 0.08, 0.0), dx, 1)
+    body = CP.RigidBody(2.0).with_position((0.45, 0.15, 0.0)).with_velocity((-0.1, 0.0, 0.0)).with_inertia(
+        CP.inertia_solid_sphere(2.0, 0.08))
+    for s in coupler.samples:
+        r = CP.quat_rotate(body.orientation, s.local)
+        parts.x[s.particle] = np.asarray(body.position) + r
+        parts.v[s.particle] = body.velocity
+    params = pef.WcsphParams.water(1000.0, 30.0, (0.0, 0.0, 0.0))
+    params.adapt_sound_speed = False
+    params.xsph_epsilon = 0.0
+    params.visc_alpha = 0.0
+    w = CP.CoupledWcsphWorld(pef.WcsphWorld(parts, params), [body], coupler, (0.0, 0.0, 0.0))
+    w.particles = parts
+    w.sync_from_rigid()
+    p0 = CP.coupled_linear_momentum(w)
+    for _ in range(50):
+        w.step(2e-4)
+    drift = np.linalg.norm(CP.coupled_linear_momentum(w) - p0)
+    print(f"momentum drift {drift:.3e}")
+    assert drift < 1e-3
+
+
+def test_floating_box_no_penetration():
+    w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    max_pen = 0.0
+    for _ in range(800):
+        w.step(2.5e-4)
+        max_pen = max(max_pen, CP.max_box_penetration_wcsph(w, (HX, HY, 0.0)))
+    print(f"max_box_penetration={max_pen:.5f}")
+    assert max_pen < 2.0 * DX
+
+
+def _submerged_sphere_3d(dx, rho_s, radius, tank):
+    """make_submerged_sphere_3d (coupling_oracles.rs:188-257)."""
+    hsm, rho0 = 2.2 * dx, 1000.0
+    m = pef.mass_calibrated_to_rest(rho0, dx, hsm, pef.D3)
+    parts = pef.Particles(pef.D3, rho0, m, hsm)
+    n = max(int(math.floor(tank / dx + 0.5)), 8)
+    pef.fill_box(parts, (0.5 * dx, 0.5 * dx, 0.5 * dx), n, n, n, dx)
+    pef.fill_box_boundary(parts, (0.0, 0.0, 0.0), (n * dx, n * dx, n * dx), dx, 1)
+    pef.recomputed_akinci_psi(parts)
+    coupler = CP.FluidRigidCoupler(1)
+    coupler.sample_sphere(parts, 0, radius, dx)
+    idx = np.array([s.particle for s in coupler.samples])
+    parts.n_out[idx] = 0.0
+    mass_b = rho_s * (4.0 / 3.0 * math.pi * radius * radius * radius)
+    c = 0.5 * n * dx
+    body = CP.RigidBody(mass_b).with_position((c, c, c)).with_inertia(CP.inertia_solid_sphere(mass_b, radius))
+    parts.x[idx] = np.asarray(body.position) + np.array([s.local for s in coupler.samples])
+    keep = ~((parts.kind == pef.FLUID) & (np.linalg.norm(parts.x - np.asarray(body.position), axis=1)
+                                          < radius + 0.5 * dx))
+    remap = np.cumsum(keep) - 1
+    fl = parts.kind[keep] == pef.FLUID
+    kept = pef.Particles(pef.D3, rho0, m, hsm)
+    kept._append(parts.x[keep], np.zeros((int(keep.sum()), 3)), parts.kind[keep],
+                 np.where(fl, 0.0, parts.psi[keep]), np.zeros((int(keep.sum()), 3)))
+    for s in coupler.samples:
+        s.particle = int(remap[s.particle])
+    params = pef.WcsphParams.water(rho0, 35.0, (0.0, -G, 0.0))
+    params.adapt_sound_speed = False
+    params.xsph_epsilon = 0.08
+    params.visc_alpha = 0.35
+    w = CP.CoupledWcsphWorld(pef.WcsphWorld(kept, params), [body], coupler, (0.0, -G, 0.0))
+    w.particles = kept
+    return w
+
+
+def test_sphere_freefall_added_mass_accel():
+    dx, rho_s, radius = 0.055, 2200.0, 0.12
+    w = _submerged_sphere_3d(dx, rho_s, radius, 0.50)
+    dt = 1.2e-4
+    for _ in range(1000):  # kinematic hold
+        w.sync_from_rigid()
+        w.fluid.step(dt)
+        w.rigid.v[0] = (0.0, 0.0, 0.0)
+        w.rigid.l[0] = (0.0, 0.0, 0.0)
+    p = w.fluid.get_params()
+    p.visc_alpha, p.xsph_epsilon = 0.01, 0.0
+    w.fluid.set_params(p)
+    w.rigid.v[0] = (0.0, 0.0, 0.0)
+    t_meas = 0.0025
+    for _ in range(int(math.floor(t_meas / dt + 0.5))):
+        w.step(dt)
+    a_dyn = -w.rigid.v[0][1] / t_meas
+    a_ref = CP.sphere_freefall_accel_added_mass(rho_s, 1000.0, G)
+    rel = abs(a_dyn - a_ref) / a_ref
+    print(f"sphere freefall a_dyn={a_dyn:.4f} a_ref={a_ref:.4f} rel={rel:.4f}")
+    assert rel < 0.10
