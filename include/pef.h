@@ -243,6 +243,55 @@ int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEn
  * and number of list rebuilds since creation. */
 int32_t pef_world_list_stats(PefWorld* world, double* mean_entries, uint64_t* rebuilds);
 
+/* ---- DFSPH (crates/pe-fluid/src/dfsph.rs) on the same grid, lists and kernels */
+
+/* DfsphParams (dfsph.rs:33-98) plus the Particles scalars. */
+typedef struct PefDfsphParams {
+  int32_t dim;                   /* 2 or 3 */
+  int32_t kernel;                /* PEF_KERNEL_CUBIC_SPLINE only */
+  double rest_density, mass, h;  /* Particles scalars */
+  uint32_t max_density_iters, max_divergence_iters;
+  double density_error_tol;      /* mean relative density error of the density solver */
+  double divergence_error_tol;   /* mean |D rho / Dt| / rho0 of the divergence solver */
+  double gravity[3];
+  double xsph_epsilon;
+  double surface_tension_sigma;  /* 0 only: PEF_ERR_UNSUPPORTED otherwise */
+  double particle_diameter;      /* 0: h (dfsph.rs:141-144) */
+  double cfl_lambda;
+  int32_t adaptive_dt;           /* kept for the caller; pef_dfsph_step uses the dt it is given */
+  int32_t warm_start;
+  int32_t free_slip_on[4];       /* planes x >= xmin, y >= ymin, x <= xmax, y <= ymax */
+  double free_slip_at[4];
+  double solver_omega;           /* Jacobi relaxation of kappa */
+  double max_velocity_correction;/* cap on |dv| per pressure update, 0 = none */
+} PefDfsphParams;
+
+/* DfsphParams::default() (dfsph.rs:76-98) with the given Particles scalars. */
+int32_t pef_dfsph_default_params(int32_t dim, double rest_density, double mass, double h, PefDfsphParams* out);
+/* DfsphWorld::new (dfsph.rs:140-180): uploads the particles (2-D: z of x and v
+ * zeroed), rebuilds the lists, computes densities and alpha. The world takes
+ * every pef_world_* getter and setter; pef_world_step and pef_world_phase
+ * (WCSPH) return PEF_ERR_UNSUPPORTED on it. Pressures read kappa
+ * (dfsph.rs:762). One scene. */
+int32_t pef_dfsph_create(const PefDfsphParams* params, const PefParticles* particles, const PefOptions* options,
+                         PefWorld** out);
+/* DfsphWorld::step(dt) (dfsph.rs:713-767) repeated nsteps times: predicted
+ * velocities, XSPH on them, the constant-density solver, drift and free-slip
+ * planes, lists and densities and alpha of the new positions, the
+ * divergence-free solver, free-slip planes, v = v*. Synchronous. Boundary
+ * reactions (dfsph.rs:490-522) are not computed: pef_world_get_boundary_force
+ * reads zeros. */
+int32_t pef_dfsph_step(PefWorld* world, double dt, uint32_t nsteps);
+#define PEF_DFSPH_PHASE_REBUILD 0  /* rebuild_neighbours (dfsph.rs:199-209) */
+#define PEF_DFSPH_PHASE_DENSITY 1  /* compute_density (dfsph.rs:212-239) */
+#define PEF_DFSPH_PHASE_ALPHA 2    /* compute_alpha (dfsph.rs:247-299) */
+int32_t pef_dfsph_phase(PefWorld* world, int32_t phase);
+/* Iterations of the last step's constant-density and divergence-free solvers
+ * (the values correct_density_error / correct_divergence_error return). */
+int32_t pef_dfsph_iterations(PefWorld* world, uint32_t* density_iters, uint32_t* divergence_iters);
+/* Per-particle alpha (dfsph.rs:247-299), original order. */
+int32_t pef_dfsph_get_alpha(PefWorld* world, double* out, uint64_t len);
+
 /* ---- rigid-body samples: the coupler's side of the step (coupling.rs:267-331) */
 
 /* Register the FluidRigidCoupler samples (coupling.rs:267-286): sample k is

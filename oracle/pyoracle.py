@@ -52,6 +52,27 @@ class OraDiag(C.Structure):
 _lib = None
 
 
+class OraDfsphParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("rest_density", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("max_density_iters", C.c_int32),
+        ("max_divergence_iters", C.c_int32),
+        ("density_error_tol", C.c_double),
+        ("divergence_error_tol", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("xsph_eps", C.c_double),
+        ("particle_diameter", C.c_double),
+        ("warm_start", C.c_int32),
+        ("slip_on", C.c_int32 * 4),
+        ("slip_at", C.c_double * 4),
+        ("solver_omega", C.c_double),
+        ("max_velocity_correction", C.c_double),
+    ]
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -115,6 +136,13 @@ def lib():
         "ora_fnv1a_bytes": (u64, [vp, u64]),
         "ora_dfsph_tank_hash": (u64, [u64]),
         "ora_coupling_tank_hash": (u64, [u64, vp]),
+        "ora_dfsph_new": (vp, [C.POINTER(OraDfsphParams), u64, vp, vp, vp, vp, vp]),
+        "ora_dfsph_free": (None, [vp]),
+        "ora_dfsph_step": (None, [vp, d, u64]),
+        "ora_dfsph_iterations": (None, [vp, vp, vp]),
+        "ora_dfsph_get_state": (None, [vp, vp, vp, vp, vp, vp, vp]),
+        "ora_dfsph_set_state": (None, [vp, vp, vp]),
+        "ora_dfsph_refresh": (None, [vp]),
         "ora_world_set_visc_xsph": (None, [vp, d, d]),
         "ora_rigid_new": (vp, [C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
         "ora_rigid_free": (None, [vp]),
@@ -203,6 +231,64 @@ def mass_calibrated_to_rest(rho0, dx, h, dim=3):
 def dfsph_tank_hash(steps=40):
     """particles_hash64 of the canonical DFSPH tank after `steps` steps (scenarios.rs:72-125)."""
     return lib().ora_dfsph_tank_hash(steps)
+
+
+def dfsph_params(dim=3, rest_density=1000.0, mass=1.0, h=0.1, max_density_iters=50, max_divergence_iters=50,
+                 density_error_tol=0.001, divergence_error_tol=0.001, gravity=(0.0, 0.0, -9.80665), xsph_eps=0.05,
+                 particle_diameter=0.0, warm_start=True, free_slip=(None, None, None, None), solver_omega=1.0,
+                 max_velocity_correction=0.0):
+    """DfsphParams (dfsph.rs:33-98); free_slip = (xmin, ymin, xmax, ymax), None = off."""
+    p = OraDfsphParams()
+    p.dim, p.rest_density, p.mass, p.h = dim, rest_density, mass, h
+    p.max_density_iters, p.max_divergence_iters = max_density_iters, max_divergence_iters
+    p.density_error_tol, p.divergence_error_tol = density_error_tol, divergence_error_tol
+    p.gravity[:] = [float(g) for g in gravity]
+    p.xsph_eps, p.particle_diameter, p.warm_start = xsph_eps, particle_diameter, 1 if warm_start else 0
+    for k, b in enumerate(free_slip):
+        p.slip_on[k] = 0 if b is None else 1
+        p.slip_at[k] = 0.0 if b is None else float(b)
+    p.solver_omega, p.max_velocity_correction = solver_omega, max_velocity_correction
+    return p
+
+
+class Dfsph:
+    """Oracle DfsphWorld (dfsph.rs:100-800) over flat f64 arrays."""
+
+    def __init__(self, params: OraDfsphParams, x, kind, v=None, psi=None, n_out=None):
+        self.n = len(kind)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        kind = np.ascontiguousarray(kind, dtype=np.uint8)
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        n_out = None if n_out is None else np.ascontiguousarray(n_out, dtype=np.float64).reshape(-1)
+        self._prm = params
+        self.hnd = lib().ora_dfsph_new(C.byref(params), self.n, _p(x), _p(v), _p(kind), _p(psi), _p(n_out))
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_dfsph_free(self.hnd)
+
+    def step(self, dt, nsteps=1):
+        lib().ora_dfsph_step(self.hnd, dt, nsteps)
+
+    def iterations(self):
+        a, b = C.c_int32(), C.c_int32()
+        lib().ora_dfsph_iterations(self.hnd, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def state(self):
+        n = self.n
+        out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), rho=np.zeros(n), p=np.zeros(n), alpha=np.zeros(n),
+                   boundary_force=np.zeros((n, 3)))
+        lib().ora_dfsph_get_state(self.hnd, *(_p(out[k]) for k in ("x", "v", "rho", "p", "alpha", "boundary_force")))
+        return out
+
+    def set_state(self, x=None, v=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        lib().ora_dfsph_set_state(self.hnd, _p(f(x)), _p(f(v)))
+
+    def refresh(self):
+        lib().ora_dfsph_refresh(self.hnd)
 
 
 def coupling_tank_hash(steps=80):

@@ -837,6 +837,7 @@ struct Dfsph {
   Grid grid;
   DfsphParams prm;
   double t = 0.0, last_dt = 0.0;
+  int last_density_iters = 0, last_divergence_iters = 0;
   std::vector<double> alpha, kappa, kappa_v, kappa_warm, kappa_v_warm;
   std::vector<V3> v_star, a_adv, scratch_dv, dv_bound, boundary_force;
   std::vector<std::pair<uint64_t, double>> reaction_tmp;
@@ -1141,7 +1142,7 @@ struct Dfsph {
       if (P.dim == DIM2) v_star[i].z = 0.0;
     }
     apply_xsph_on_vstar();
-    correct_density_error(dt);
+    last_density_iters = correct_density_error(dt);
     for (size_t i = 0; i < n; ++i) {
       if (!fluid(i)) continue;
       P.x[i] = add(P.x[i], mul(v_star[i], dt));
@@ -1151,7 +1152,7 @@ struct Dfsph {
     rebuild_neighbours();
     compute_density();
     compute_alpha();
-    correct_divergence_error(dt);
+    last_divergence_iters = correct_divergence_error(dt);
     apply_free_slip_walls();
     for (size_t i = 0; i < n; ++i) {
       if (!fluid(i)) continue;
@@ -1803,6 +1804,93 @@ void ora_world_free(void* w) { delete (World*)w; }
 void ora_world_set_visc_xsph(void* w, double visc_alpha, double xsph_eps) {
   ((World*)w)->params.visc.alpha = visc_alpha;
   ((World*)w)->params.xsph_eps = xsph_eps;
+}
+
+// --- DFSPH world (dfsph.rs) ----------------------------------------------------
+typedef struct {
+  int32_t dim;
+  double rest_density, mass, h;
+  int32_t max_density_iters, max_divergence_iters;
+  double density_error_tol, divergence_error_tol;
+  double gravity[3];
+  double xsph_eps, particle_diameter;
+  int32_t warm_start;
+  int32_t slip_on[4];
+  double slip_at[4];
+  double solver_omega, max_velocity_correction;
+} OraDfsphParams;
+
+// DfsphWorld::new (dfsph.rs:140-180) over flat arrays (kind 0 fluid, 1 boundary).
+void* ora_dfsph_new(const OraDfsphParams* prm, uint64_t n, const double* x, const double* v, const uint8_t* kind,
+                    const double* psi, const double* n_out) {
+  Dfsph* D = new Dfsph();
+  D->P.dim = prm->dim;
+  D->P.rest_density = prm->rest_density;
+  D->P.mass = prm->mass;
+  D->P.h = prm->h;
+  for (size_t i = 0; i < n; ++i) {
+    const V3 px = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    if (kind[i] == KIND_FLUID)
+      D->P.push_fluid(px, v ? v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]) : v3(0, 0, 0));
+    else
+      D->P.push_boundary_oriented(px, psi ? psi[i] : 0.0,
+                                  n_out ? v3(n_out[3 * i], n_out[3 * i + 1], n_out[3 * i + 2]) : v3(0, 0, 0));
+  }
+  DfsphParams& q = D->prm;
+  q.max_density_iters = prm->max_density_iters;
+  q.max_divergence_iters = prm->max_divergence_iters;
+  q.density_error_tol = prm->density_error_tol;
+  q.divergence_error_tol = prm->divergence_error_tol;
+  q.gravity = v3(prm->gravity[0], prm->gravity[1], prm->gravity[2]);
+  q.xsph_eps = prm->xsph_eps;
+  q.particle_diameter = prm->particle_diameter;
+  q.warm_start = prm->warm_start != 0;
+  for (int k = 0; k < 4; ++k) {
+    q.slip_on[k] = prm->slip_on[k] != 0;
+    q.slip_at[k] = prm->slip_at[k];
+  }
+  q.solver_omega = prm->solver_omega;
+  q.max_velocity_correction = prm->max_velocity_correction;
+  D->init();
+  return D;
+}
+void ora_dfsph_free(void* d) { delete (Dfsph*)d; }
+void ora_dfsph_step(void* d, double dt, uint64_t nsteps) {
+  for (uint64_t k = 0; k < nsteps; ++k) ((Dfsph*)d)->step(dt);
+}
+void ora_dfsph_iterations(void* d, int32_t* dens, int32_t* div) {
+  *dens = ((Dfsph*)d)->last_density_iters;
+  *div = ((Dfsph*)d)->last_divergence_iters;
+}
+// x, v, rho, p (= kappa), alpha, boundary_force; any may be null
+void ora_dfsph_get_state(void* d, double* x, double* v, double* rho, double* p, double* alpha, double* bforce) {
+  const Dfsph& D = *(Dfsph*)d;
+  for (size_t i = 0; i < D.P.len(); ++i) {
+    const double xs[3] = {D.P.x[i].x, D.P.x[i].y, D.P.x[i].z}, vs[3] = {D.P.v[i].x, D.P.v[i].y, D.P.v[i].z};
+    if (x) std::memcpy(x + 3 * i, xs, 24);
+    if (v) std::memcpy(v + 3 * i, vs, 24);
+    if (rho) rho[i] = D.P.rho[i];
+    if (p) p[i] = D.P.p[i];
+    if (alpha) alpha[i] = D.alpha[i];
+    if (bforce) {
+      const double f[3] = {D.boundary_force[i].x, D.boundary_force[i].y, D.boundary_force[i].z};
+      std::memcpy(bforce + 3 * i, f, 24);
+    }
+  }
+}
+void ora_dfsph_set_state(void* d, const double* x, const double* v) {
+  Dfsph& D = *(Dfsph*)d;
+  for (size_t i = 0; i < D.P.len(); ++i) {
+    if (x) D.P.x[i] = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    if (v) D.P.v[i] = v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  }
+}
+// rebuild_neighbours, compute_density, compute_alpha (dfsph.rs:199-299)
+void ora_dfsph_refresh(void* d) {
+  Dfsph& D = *(Dfsph*)d;
+  D.rebuild_neighbours();
+  D.compute_density();
+  D.compute_alpha();
 }
 
 // --- rigid world (pe-core) ---------------------------------------------------

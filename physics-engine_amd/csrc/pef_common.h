@@ -9,7 +9,7 @@
 //             exactly; w = +m for fluid, -rho0*psi for boundary (sign bit =
 //             boundary)
 //   A2        f32x4 same positions, w = rho for fluid (written by density)
-//   B         f32x4 (vx, vy, vz, -) velocities
+//   B         f32x4 (vx, vy, vz, -) velocities (DFSPH: w = kappa, the pressure)
 //   B2        f32x4 (vx, vy, vz, P) velocities after XSPH, P = p / rho^2
 //   ID        u32  original particle index (global across scenes)
 //   META      u32  bit0 boundary, bit1 oriented boundary, bits 8.. scene index
@@ -61,6 +61,7 @@ struct SceneDev {
   int32_t oriented;   // scene holds one-sided boundary samples
   int32_t bricked;    // 1: brick-major cell ids (list grid); 0: (ix, iy, iz) lexicographic (reference grid)
   int32_t nbx, nby;   // bricks along x and y (bricked grids; bricks along z = nz)
+  int32_t dfsph;      // the scene steps with DFSPH: no Tait pressure; B.w holds kappa
   float cell_f;
   float h, inv_h, mass, rho0, inv_rho0;
   float alpha;       // cubic spline normalisation

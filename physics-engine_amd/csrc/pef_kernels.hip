@@ -212,6 +212,7 @@ __global__ void k_finalize(uint32_t N, const Ctl* __restrict__ ctl, const SceneD
     alt.NY[dst] = cur.NY[s];
     alt.NZ[dst] = cur.NZ[s];
   }
+  if (cur.DW) alt.DW[dst] = cur.DW[s];
 }
 
 __global__ void k_copyback(uint32_t N, const Ctl* __restrict__ ctl, State alt, State cur,
@@ -237,6 +238,7 @@ __global__ void k_copyback(uint32_t N, const Ctl* __restrict__ ctl, State alt, S
     cur.NY[s] = alt.NY[s];
     cur.NZ[s] = alt.NZ[s];
   }
+  if (cur.DW) cur.DW[s] = alt.DW[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -424,7 +426,8 @@ __global__ void k_unsort(uint32_t N, const SceneDev* __restrict__ sc, const Scen
       out[id] = bnd ? S.rho0_d : (double)cur.A2[s].w;
       break;
     case 4:
-      out[id] = bnd ? 0.0 : (double)tait_p(cur.A2[s].w, S, (float)st[scene_of(meta)].B);
+      if (S.dfsph) out[id] = bnd ? 0.0 : (double)cur.B[s].w;  // kappa (dfsph.rs:762)
+      else out[id] = bnd ? 0.0 : (double)tait_p(cur.A2[s].w, S, (float)st[scene_of(meta)].B);
       break;
     case 5: {
       float4 f = cur.BF[s];

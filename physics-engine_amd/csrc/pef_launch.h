@@ -17,6 +17,7 @@ struct State {
   float* DISP;
   double *NX, *NY, *NZ;
   float4 *ACC, *BF;
+  float2* DW;  // DFSPH warm-start sums (kappa, kappa_v), travels with the particle; null otherwise
 };
 
 struct GridArgs {

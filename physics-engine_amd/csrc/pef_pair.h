@@ -11,7 +11,7 @@ namespace pef {
 enum { kTileNlist = 0, kTileDensity = 1, kTileXsph = 2, kTileForce = 3, kTileKinds = 4 };
 // The surrogate feature pass has its own queue block (16 words) after the
 // four kinds'; it uses the force pass's LDS layout.
-enum { kTileFeatures = 4, kTileQueues = 5 };
+enum { kTileFeatures = 4, kTileDfAlpha = 5, kTileDfKappa = 6, kTileDfUpdate = 7, kTileQueues = 8 };
 constexpr int kSurShells = 5, kSurFeatures = 32;
 constexpr uint32_t kFeatureThreads = 512;
 
@@ -68,6 +68,57 @@ uint32_t sur_check_blocks(uint32_t N);
 // four FP64 partial sums per block (see k_sur_check)
 void launch_sur_check(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const float4* pred,
                       double* part);
+// DFSPH solver loop control on the device: a pass of an iteration that starts
+// after the loop ended returns at once, so the host can enqueue iterations
+// ahead of the convergence test.
+struct DfCtl {
+  uint32_t done;    // the loop is over
+  uint32_t stop;    // the last reduction met the stopping rule (its update still runs)
+  uint32_t iter;    // iterations run
+  uint32_t arrive;  // workgroups finished in the reduction
+  double avg;       // mean error of the last iteration
+};
+struct DfArgs {
+  DfCtl* ctl;
+  double* ERR;  // per slot: the iteration's error term / rho0 (0 for boundary)
+  float* AL;    // per slot: alpha
+  float dt, inv_dt, inv_dt2, omega, cap;
+  int div;   // 1: divergence solver, 0: density solver
+  int warm;  // accumulate the warm-start sums (State::DW)
+};
+struct DfTiles {
+  uint32_t lds_cap[3];  // alpha, stiffness, update
+  uint32_t grid[3];
+};
+int df_configure(const TileConfig& c, DfTiles& d);
+void launch_df_alpha(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                     float* AL, float floor_);
+void launch_df_sparse_alpha(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur,
+                            const uint32_t* cell_start, float* AL, double inv_ref);
+void launch_df_kappa(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                     const DfArgs& da);
+void launch_df_update(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                      const DfArgs& da);
+
+// Free-slip planes x >= xmin, y >= ymin, x <= xmax, y <= ymax (dfsph.rs:660-711).
+struct DfSlip {
+  int32_t on[4];
+  double at[4];
+};
+// Elementwise DFSPH passes (pef_dfsph.hip).
+void launch_df_begin(hipStream_t st, DfCtl* c);
+void launch_df_predict(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, float dt);
+void launch_df_warm_prep(hipStream_t st, uint32_t N, const State& cur, int which);
+void launch_df_warm_zero(hipStream_t st, uint32_t N, const State& cur, int which);
+uint32_t df_reduce_blocks(uint32_t N);
+void launch_df_reduce(hipStream_t st, uint32_t N, const double* ERR, double* part, DfCtl* c, double count, double tol,
+                      uint32_t min_iter, uint32_t max_iter);
+void launch_df_drift(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, Ctl* ctlw, double dt,
+                     const DfSlip& slip);
+void launch_df_slip(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, Ctl* ctlw, const DfSlip& slip);
+void launch_df_finish(hipStream_t st, uint32_t N, const State& cur);
+void launch_df_unsort_f32(hipStream_t st, uint32_t N, const State& cur, const float* src, double* out);
+
 void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint16_t* LIST,
                          const uint32_t* CNT, uint32_t cap, const uint32_t* cell_start, uint64_t* out,
                          uint32_t* out_len);

@@ -571,7 +571,8 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
     const float r2s = S.r2s_u, qu = S.q_unit;
-    const bool eos = S.adapt == 0;  // fixed sound speed: the pressure needs no global reduction first
+    // fixed sound speed: the pressure needs no global reduction first (DFSPH has no EOS)
+    const bool eos = S.adapt == 0 && S.dfsph == 0;
     const float Bf = (float)(S.rho0_d * st[Tb.scene].c * st[Tb.scene].c / S.gamma_d);
     const Org org = org_of(Tb);
     const Src<true, 1> fsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
@@ -994,6 +995,321 @@ __global__ void k_sur_check(uint32_t N, const SceneDev* __restrict__ sc, State c
 }
 
 // ---------------------------------------------------------------------------
+// DFSPH pair passes (dfsph.rs:247-528) over the same lists and halos. Pair
+// gradient: grad W_ij = (x_j - x_i) gf with gf = -dW/dr / r, the reference's
+// grad_w(x_j - x_i) (kernels.rs:56-75). Partner weight m_j = |A.w|: m for
+// fluid, rho0 psi for a boundary sample the list kept (boundary_sees_fluid at
+// the build); the sentinel weighs 0. Only fluid particles are evaluated.
+// ---------------------------------------------------------------------------
+constexpr int kDfLpp = 4;
+constexpr int kDfThreads = 512;
+
+// alpha_i = rho_i / max(|sum m_j grad W|^2 + sum |m_j grad W|^2, floor), 0 with
+// fewer than two candidates (dfsph.rs:247-299). The reference counts the
+// candidates of its 27 cells of edge 2h (fluid, and boundary samples that see
+// the particle and weigh more than 0), which hold every partner within 2h: a
+// particle with two such partners has its alpha here; one with fewer is marked
+// (alpha stored negated) for k_df_sparse_alpha, which counts its candidates.
+template <bool FAST>
+__device__ inline float4 df_alpha_walk(const Round& r, uint32_t sub, const float4& ai, float qu, float aoh, float r2s,
+                                       const Src<FAST, 1>& src, uint32_t& near) {
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, sq = 0.0f;
+  uint32_t n = 0;
+  walk_chunks<kDfLpp, 2>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
+    float4 a[8], unused;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) src.get(e[k], a[k], unused);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float4 aj = a[k];
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+      const float g = -cubic_dwdr(r2 * rinv * qu, aoh) * rinv * fabsf(aj.w);
+      const float gx = rx * g, gy = ry * g, gz = rz * g;
+      sx += gx;
+      sy += gy;
+      sz += gz;
+      sq += gx * gx + gy * gy + gz * gz;
+      n += (r2 < r2s && aj.w != 0.0f) ? 1u : 0u;
+    }
+  });
+  near = n;
+  return make_float4(sx, sy, sz, sq);
+}
+
+__global__ void __launch_bounds__(kDfThreads) k_df_alpha_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
+                                                                State cur, float* __restrict__ AL, float floor_) {
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  float4* R = lds;
+  const uint32_t CAP = ta.lds_cap;
+  sentinel_lds<1>(R, CAP);
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / kDfLpp, sub = threadIdx.x % kDfLpp;
+    Round rd = round_load<kDfLpp, 2>(Tb, ta, threadIdx.x / kDfLpp, n_own, sub);
+    if (fast) stage_halo<1>(Tb, H, R, CAP, cur.A, nullptr);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const float qu = S.q_unit, aoh = S.alpha * S.inv_h, r2s = S.r2s_u;
+    const Org org = org_of(Tb);
+    const Src<true, 1> fsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
+    const Src<false, 1> gsrc{R, CAP, cur.A, nullptr, &Tb, ta.N, org};
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      if (base != 0) rd = round_load<kDfLpp, 2>(Tb, ta, base + threadIdx.x / kDfLpp, n_own, sub);
+      const Own o = rd.o;
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), unused, g = ai;
+      uint32_t near = 0;
+      bool fl = false;
+      if (rd.act) {
+        if (fast) fsrc.own(o.local, o.s, ai, unused); else gsrc.own(o.local, o.s, ai, unused);
+        fl = !is_boundary_w(ai.w);
+        if (fl)
+          g = fast ? df_alpha_walk<true>(rd, sub, ai, qu, aoh, r2s, fsrc, near)
+                   : df_alpha_walk<false>(rd, sub, ai, qu, aoh, r2s, gsrc, near);
+      }
+      g.x = group_sum<kDfLpp>(g.x);
+      g.y = group_sum<kDfLpp>(g.y);
+      g.z = group_sum<kDfLpp>(g.z);
+      g.w = group_sum<kDfLpp>(g.w);
+      near = (uint32_t)group_sum<kDfLpp>((float)near);
+      if (rd.act && sub == 0u) {
+        float a = 0.0f;
+        if (fl) {
+          a = cur.A2[o.s].w / fmaxf(g.x * g.x + g.y * g.y + g.z * g.z + g.w, floor_);
+          if (near < 2u) a = -a;  // k_df_sparse_alpha decides
+        }
+        AL[o.s] = a;
+      }
+    }
+  });
+}
+
+// The candidates of a marked particle in the reference grid (cell edge 2h,
+// keys floor(x / 2h), grid.rs:59-70): the particles of its 27 cells but
+// itself, fluid or boundary samples that weigh more than 0 and see it
+// (dfsph.rs:262-281). They lie within 4h in each axis, so the list grid's
+// 5 x 5 x 5 cells around the particle hold them all. Fewer than two: alpha = 0.
+__global__ void k_df_sparse_alpha(uint32_t N, const SceneDev* __restrict__ sc, State cur,
+                                  const uint32_t* __restrict__ cell_start, float* __restrict__ AL, double inv_ref) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const float a = AL[s];
+  if (!(a < 0.0f)) return;
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  const double xi = cur.X[s], yi = cur.Y[s], zi = cur.Z[s];
+  const int32_t kx = floor_i32(xi * inv_ref), ky = floor_i32(yi * inv_ref);
+  const int32_t kz = S.dim == 2 ? 0 : floor_i32(zi * inv_ref);
+  const int64_t lx = (int64_t)floor_i32(xi * S.inv_cell) - S.ix0, ly = (int64_t)floor_i32(yi * S.inv_cell) - S.iy0;
+  const int64_t lz = S.dim == 2 ? 0 : (int64_t)floor_i32(zi * S.inv_cell) - S.iz0;
+  const int64_t dz0 = S.dim == 2 ? 0 : -2, dz1 = S.dim == 2 ? 0 : 2;
+  uint32_t n = 0;
+  for (int64_t dx = -2; dx <= 2 && n < 2u; ++dx)
+    for (int64_t dy = -2; dy <= 2 && n < 2u; ++dy)
+      for (int64_t dz = dz0; dz <= dz1 && n < 2u; ++dz) {
+        const int64_t cx = lx + dx, cy = ly + dy, cz = lz + dz;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= S.nx || cy >= S.ny || cz >= S.nz) continue;
+        const uint32_t c = cell_id(S, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        for (uint32_t j = cell_start[c]; j < cell_start[c + 1] && n < 2u; ++j) {
+          if (j == s) continue;
+          const double xj = cur.X[j], yj = cur.Y[j], zj = cur.Z[j];
+          const int32_t jx = floor_i32(xj * inv_ref), jy = floor_i32(yj * inv_ref);
+          const int32_t jz = S.dim == 2 ? 0 : floor_i32(zj * inv_ref);
+          if (abs(jx - kx) > 1 || abs(jy - ky) > 1 || abs(jz - kz) > 1) continue;
+          const uint32_t meta = cur.META[j];
+          if (meta & kMetaBoundary) {
+            if (cur.A[j].w == 0.0f) continue;  // psi = 0: no mass
+            if ((meta & kMetaOriented) &&
+                !sees_f64(xj, yj, zj, cur.NX[j], cur.NY[j], cur.NZ[j], xi, yi, zi, S.h_d))
+              continue;
+          }
+          ++n;
+        }
+      }
+  AL[s] = n >= 2u ? -a : 0.0f;
+}
+
+// One solver iteration's stiffness (dfsph.rs:318-353, 519-621): the density
+// derivative sum m_j (v*_i - v*_j) . grad W on the predicted velocities (B2;
+// a boundary partner's is its own velocity), then
+//   density solver:    err = max(rho + dt drho - rho0, 0), kappa = w err / dt^2 alpha
+//   divergence solver: err = max(drho, 0),                  kappa = w err / dt alpha
+// per fluid particle: ERR = err / rho0 (FP64, summed by k_df_reduce), B2.w =
+// kappa / max(rho, 1e-12) for the update pass, the warm-start sum, and the
+// density solver's kappa in B.w (the pressure the step reports).
+template <bool FAST>
+__device__ inline float df_drho_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi, float qu,
+                                     float aoh, const Src<FAST, 2>& src) {
+  float d = 0.0f;
+  walk_chunks<kDfLpp, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
+    float4 a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) src.get(e[k], a[k], b[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float4 aj = a[k], bj = b[k];
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+      const float g = -cubic_dwdr(r2 * rinv * qu, aoh) * rinv * fabsf(aj.w);
+      d += ((bi.x - bj.x) * rx + (bi.y - bj.y) * ry + (bi.z - bj.z) * rz) * g;
+    }
+  });
+  return d;
+}
+
+__global__ void __launch_bounds__(kDfThreads) k_df_kappa_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
+                                                                State cur, DfArgs da) {
+  if (da.ctl->done | da.ctl->stop) {  // converged: the loop is over
+    if (blockIdx.x == 0 && threadIdx.x == 0) da.ctl->done = 1u;
+    return;
+  }
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  float4* R = lds;
+  const uint32_t CAP = ta.lds_cap;
+  sentinel_lds<2>(R, CAP);
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / kDfLpp, sub = threadIdx.x % kDfLpp;
+    Round rd = round_load<kDfLpp, 1>(Tb, ta, threadIdx.x / kDfLpp, n_own, sub);
+    if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A, cur.B2);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const float qu = S.q_unit, aoh = S.alpha * S.inv_h;
+    const Org org = org_of(Tb);
+    const Src<true, 2> fsrc{R, CAP, cur.A, cur.B2, &Tb, ta.N, org};
+    const Src<false, 2> gsrc{R, CAP, cur.A, cur.B2, &Tb, ta.N, org};
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      if (base != 0) rd = round_load<kDfLpp, 1>(Tb, ta, base + threadIdx.x / kDfLpp, n_own, sub);
+      const Own o = rd.o;
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
+      float d = 0.0f;
+      bool fl = false;
+      if (rd.act) {
+        if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
+        fl = !is_boundary_w(ai.w);
+        if (fl) d = fast ? df_drho_walk<true>(rd, sub, ai, bi, qu, aoh, fsrc) : df_drho_walk<false>(rd, sub, ai, bi, qu, aoh, gsrc);
+      }
+      d = group_sum<kDfLpp>(d);
+      if (rd.act && sub == 0u) {
+        float* b2w = reinterpret_cast<float*>(cur.B2) + 4ull * o.s + 3;
+        if (!fl) {
+          da.ERR[o.s] = 0.0;
+          *b2w = 0.0f;
+        } else {
+          const float rho = cur.A2[o.s].w;
+          const float err = da.div ? fmaxf(d, 0.0f) : fmaxf(rho + da.dt * d - S.rho0, 0.0f);
+          const float kappa = (da.div ? da.omega * err * da.inv_dt : da.omega * err * da.inv_dt2) * da.AL[o.s];
+          da.ERR[o.s] = (double)err / S.rho0_d;
+          *b2w = kappa / fmaxf(rho, 1e-12f);
+          if (!da.div) reinterpret_cast<float*>(cur.B)[4ull * o.s + 3] = kappa;
+          if (da.warm) {
+            float* w = reinterpret_cast<float*>(cur.DW) + 2ull * o.s + (da.div ? 1 : 0);
+            *w = *w + kappa;
+          }
+        }
+      }
+    }
+  });
+}
+
+// Pressure velocity update (dfsph.rs:423-528), Jacobi: dv_i = -dt sum m_j
+// (K_i + K_j) grad W with K = kappa / rho from the stiffness pass (0 for a
+// boundary partner), capped at max_velocity_correction, added to v* in place
+// (the pass reads no partner velocity).
+template <bool FAST>
+__device__ inline float3 df_update_walk(const Round& r, uint32_t sub, const float4& ai, float ki, float qu, float aoh,
+                                        const Src<FAST, 2>& src) {
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  walk_chunks<kDfLpp, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+    uint32_t e[8];
+    chunk_entries(ch, e);
+    float4 a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) src.get(e[k], a[k], b[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float4 aj = a[k], bj = b[k];
+      const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+      const float kj = (aj.w > 0.0f) ? bj.w : 0.0f;
+      const float g = -cubic_dwdr(r2 * rinv * qu, aoh) * rinv * fabsf(aj.w) * (ki + kj);
+      x += rx * g;
+      y += ry * g;
+      z += rz * g;
+    }
+  });
+  return make_float3(x, y, z);
+}
+
+__global__ void __launch_bounds__(kDfThreads) k_df_update_tiled(TileArgs ta, const SceneDev* __restrict__ sc,
+                                                                 State cur, DfArgs da) {
+  if (da.ctl->done) return;
+  extern __shared__ float4 lds[];
+  __shared__ HaloTab T;
+  float4* R = lds;
+  const uint32_t CAP = ta.lds_cap;
+  sentinel_lds<2>(R, CAP);
+  brick_loop(ta, T, [&](const HaloTab& Tb) {
+    const uint32_t H = Tb.pre[kHaloCells];
+    const bool fast = H <= CAP;
+    const uint32_t n_own = Tb.s1 - Tb.s0;
+    const uint32_t ppr = blockDim.x / kDfLpp, sub = threadIdx.x % kDfLpp;
+    Round rd = round_load<kDfLpp, 1>(Tb, ta, threadIdx.x / kDfLpp, n_own, sub);
+    if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A, cur.B2);
+    __syncthreads();
+    const SceneDev& S = sc[Tb.scene];
+    const float qu = S.q_unit, aoh = S.alpha * S.inv_h;
+    const Org org = org_of(Tb);
+    const Src<true, 2> fsrc{R, CAP, cur.A, cur.B2, &Tb, ta.N, org};
+    const Src<false, 2> gsrc{R, CAP, cur.A, cur.B2, &Tb, ta.N, org};
+    for (uint32_t base = 0; base < n_own; base += ppr) {
+      if (base != 0) rd = round_load<kDfLpp, 1>(Tb, ta, base + threadIdx.x / kDfLpp, n_own, sub);
+      const Own o = rd.o;
+      float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
+      float3 dv = make_float3(0.0f, 0.0f, 0.0f);
+      bool fl = false;
+      if (rd.act) {
+        if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
+        fl = !is_boundary_w(ai.w);
+        if (fl)
+          dv = fast ? df_update_walk<true>(rd, sub, ai, bi.w, qu, aoh, fsrc)
+                    : df_update_walk<false>(rd, sub, ai, bi.w, qu, aoh, gsrc);
+      }
+      dv.x = group_sum<kDfLpp>(dv.x);
+      dv.y = group_sum<kDfLpp>(dv.y);
+      dv.z = group_sum<kDfLpp>(dv.z);
+      if (rd.act && sub == 0u && fl) {
+        float vx = -da.dt * dv.x, vy = -da.dt * dv.y, vz = -da.dt * dv.z;
+        const float s2 = vx * vx + vy * vy + vz * vz;
+        if (da.cap > 0.0f && s2 > da.cap * da.cap) {
+          const float sc_ = da.cap / sqrtf(s2);
+          vx *= sc_;
+          vy *= sc_;
+          vz *= sc_;
+        }
+        const float4 v = cur.B2[o.s];
+        float* b2 = reinterpret_cast<float*>(cur.B2) + 4ull * o.s;
+        b2[0] = v.x + vx;
+        b2[1] = v.y + vy;
+        b2[2] = (S.dim == 2) ? 0.0f : v.z + vz;
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
 // brick work list: non-empty bricks and their halo tables, at each list build
 // ---------------------------------------------------------------------------
 __global__ void k_brick_flags(uint32_t NB, const Ctl* __restrict__ ctl, int always,
@@ -1222,6 +1538,54 @@ uint32_t sur_check_blocks(uint32_t N) { return blocks_of(N, 256); }
 void launch_sur_check(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const float4* pred,
                       double* part) {
   PEF_LAUNCH_LDS(k_sur_check, blocks_of(N, 256), 256, 0, st, N, sc, cur, pred, part);
+}
+
+// DFSPH passes: one instantiation (four lanes per particle, 512 threads); the
+// one-array alpha pass uses the density pass's LDS capacity, the others the
+// XSPH pass's.
+int df_configure(const TileConfig& c, DfTiles& d) {
+  const void* fns[3] = {reinterpret_cast<const void*>(&k_df_alpha_tiled),
+                        reinterpret_cast<const void*>(&k_df_kappa_tiled),
+                        reinterpret_cast<const void*>(&k_df_update_tiled)};
+  for (int k = 0; k < 3; ++k) {
+    const int narr = k == 0 ? 1 : 2;
+    d.lds_cap[k] = c.lds_cap[k == 0 ? kTileDensity : kTileXsph];
+    const size_t bytes = lds_bytes(d.lds_cap[k], narr);
+    if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 3;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[k], kDfThreads, bytes) != hipSuccess) return 3;
+    d.grid[k] = c.n_cu * (uint32_t)(per_cu < 1 ? 1 : per_cu);
+  }
+  return 0;
+}
+
+void launch_df_alpha(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                     float* AL, float floor_) {
+  TileArgs a = ta;
+  a.lds_cap = d.lds_cap[0];
+  a.queue = ta.queue + 16 * kTileDfAlpha;
+  PEF_LAUNCH_LDS(k_df_alpha_tiled, d.grid[0], kDfThreads, lds_bytes(a.lds_cap, 1), st, a, sc, cur, AL, floor_);
+}
+
+void launch_df_sparse_alpha(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur,
+                            const uint32_t* cell_start, float* AL, double inv_ref) {
+  PEF_LAUNCH_LDS(k_df_sparse_alpha, blocks_of(N, 256), 256, 0, st, N, sc, cur, cell_start, AL, inv_ref);
+}
+
+void launch_df_kappa(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                     const DfArgs& da) {
+  TileArgs a = ta;
+  a.lds_cap = d.lds_cap[1];
+  a.queue = ta.queue + 16 * kTileDfKappa;
+  PEF_LAUNCH_LDS(k_df_kappa_tiled, d.grid[1], kDfThreads, lds_bytes(a.lds_cap, 2), st, a, sc, cur, da);
+}
+
+void launch_df_update(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,
+                      const DfArgs& da) {
+  TileArgs a = ta;
+  a.lds_cap = d.lds_cap[2];
+  a.queue = ta.queue + 16 * kTileDfUpdate;
+  PEF_LAUNCH_LDS(k_df_update_tiled, d.grid[2], kDfThreads, lds_bytes(a.lds_cap, 2), st, a, sc, cur, da);
 }
 
 void launch_extract_list(hipStream_t st, uint32_t slot, const SceneDev* sc, const State& cur, const uint16_t* LIST,

@@ -103,6 +103,8 @@ class World {
            *slot_map_f = nullptr;
   double *rs_local = nullptr, *rs_localn = nullptr, *rs_pose = nullptr, *rs_out = nullptr;
   void* rs_mem = nullptr;
+  // DFSPH solver state (pef_dfsph_create); null for a WCSPH world
+  struct DfsphHost* df = nullptr;
 
   ~World() { release(); }
 
@@ -113,7 +115,8 @@ class World {
     if (e == hipSuccess) allocs.push_back((void*)*p);
     return e;
   }
-  void release() {
+  void release();
+  void release_buffers() {
     if (rs_mem) (void)hipFree(rs_mem);
     rs_mem = nullptr;
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -136,6 +139,25 @@ class World {
     probe_valid = false;
   }
 };
+
+// DFSPH parameters and device buffers (the buffers are World allocations).
+struct DfsphHost {
+  PefDfsphParams prm{};
+  DfTiles tiles{};
+  DfCtl* ctl = nullptr;
+  double* ERR = nullptr;
+  double* part = nullptr;
+  float* AL = nullptr;
+  double count = 0.0;  // fluid particles
+  float floor_ = 0.0f;
+  uint32_t last_density = 0, last_divergence = 0;
+};
+
+inline void World::release() {
+  release_buffers();
+  delete df;
+  df = nullptr;
+}
 
 }  // namespace pef
 
@@ -499,6 +521,8 @@ pef::TileArgs tile_args(World& w) {
   a.N = w.N;
   return a;
 }
+
+void launch_alpha(World& w);
 
 void launch_density(World& w) {
   pef::launch_density_tiled(w.stream, w.tc, tile_args(w), w.sc_d, w.st_d, w.cur);
@@ -1264,6 +1288,7 @@ int32_t sync_rigid(World& w, uint32_t n_bodies, const double* x, const double* q
                          w.sc_d, w.cur, w.ctl_d);
   enqueue_rebuild(w, nullptr);
   launch_density(w);
+  if (w.df) launch_alpha(w);  // the coupled DFSPH step refreshes alpha too (coupling.rs:393-399)
   HIPTRY(hipGetLastError());
   w.probe_valid = false;
   return read_flags(w);
@@ -1293,6 +1318,237 @@ int32_t reduce_rigid(World& w, uint32_t n_bodies, const double* com, double* for
       force[3 * b + a] = out[6 * b + a];
       torque[3 * b + a] = out[6 * b + 3 + a];
     }
+  return PEF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// DFSPH (dfsph.rs) on the WCSPH world's grid, lists, density and XSPH passes
+// ---------------------------------------------------------------------------
+int32_t check_dfsph_params(const PefDfsphParams& p) {
+  if (p.dim != 2 && p.dim != 3) return PEF_ERR_ARG;
+  if (p.kernel != PEF_KERNEL_CUBIC_SPLINE) return PEF_ERR_UNSUPPORTED;
+  if (p.surface_tension_sigma != 0.0) return PEF_ERR_UNSUPPORTED;
+  const double pos[] = {p.rest_density, p.mass, p.h};
+  for (double v : pos)
+    if (!(std::isfinite(v) && v > 0.0)) return PEF_ERR_ARG;
+  const double fin[] = {p.density_error_tol, p.divergence_error_tol, p.xsph_epsilon, p.particle_diameter,
+                        p.cfl_lambda, p.solver_omega, p.max_velocity_correction};
+  for (double v : fin)
+    if (!std::isfinite(v)) return PEF_ERR_ARG;
+  if (p.max_velocity_correction < 0.0) return PEF_ERR_ARG;
+  if (!finite3(p.gravity)) return PEF_ERR_ARG;
+  for (int k = 0; k < 4; ++k)
+    if (p.free_slip_on[k] && !std::isfinite(p.free_slip_at[k])) return PEF_ERR_ARG;
+  return PEF_OK;
+}
+
+pef::DfSlip df_slip(const PefDfsphParams& p) {
+  pef::DfSlip s{};
+  for (int k = 0; k < 4; ++k) {
+    s.on[k] = p.free_slip_on[k] ? 1 : 0;
+    s.at[k] = p.free_slip_at[k];
+  }
+  return s;
+}
+
+void launch_alpha(World& w) {
+  pef::launch_df_alpha(w.stream, w.df->tiles, tile_args(w), w.sc_d, w.cur, w.df->AL, w.df->floor_);
+  pef::launch_df_sparse_alpha(w.stream, w.N, w.sc_d, w.cur, w.cell_start, w.df->AL, 1.0 / (2.0 * w.params[0].h));
+}
+
+// rebuild_neighbours + compute_density + compute_alpha of the current positions
+// when the densities are not theirs.
+int32_t df_ensure(World& w) {
+  if (w.density_valid) return PEF_OK;
+  int32_t rc = set_force_rebuild(w);
+  if (rc) return rc;
+  enqueue_rebuild(w, nullptr);
+  launch_density(w);
+  launch_alpha(w);
+  HIPTRY(hipGetLastError());
+  w.density_valid = true;
+  return PEF_OK;
+}
+
+// One solver (correct_density_error, dfsph.rs:531-578, or
+// correct_divergence_error, dfsph.rs:581-621). Iterations are enqueued in
+// chunks ahead of the stopping test, which runs on the device; an iteration
+// enqueued after the loop ended returns at once. The host reads the control
+// block after each chunk; chunks double, so a solver that runs k iterations
+// waits for the device about log2(k) times.
+int32_t df_solve(World& w, double dt, bool div, uint32_t* iters) {
+  pef::DfsphHost& D = *w.df;
+  const PefDfsphParams& p = D.prm;
+  pef::DfArgs da{};
+  da.ctl = D.ctl;
+  da.ERR = D.ERR;
+  da.AL = D.AL;
+  da.dt = (float)dt;
+  da.inv_dt = (float)(1.0 / dt);
+  da.inv_dt2 = (float)(1.0 / (dt * dt));
+  da.omega = (float)p.solver_omega;
+  da.cap = (float)p.max_velocity_correction;
+  da.div = div ? 1 : 0;
+  da.warm = p.warm_start ? 1 : 0;
+  const pef::TileArgs ta = tile_args(w);
+  const int which = div ? 1 : 0;
+  pef::launch_df_begin(w.stream, D.ctl);
+  if (p.warm_start) {  // kappa = warm sum, one pressure update, warm sums cleared
+    pef::launch_df_warm_prep(w.stream, w.N, w.cur, which);
+    pef::launch_df_update(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+    pef::launch_df_warm_zero(w.stream, w.N, w.cur, which);
+  }
+  const uint32_t max_it = std::max<uint32_t>(div ? p.max_divergence_iters : p.max_density_iters, 1u);
+  const uint32_t min_it = div ? 1u : 2u;
+  const double tol = div ? p.divergence_error_tol : p.density_error_tol;
+  uint32_t queued = 0, chunk = std::min(min_it, max_it);
+  pef::DfCtl h{};
+  while (queued < max_it) {
+    for (uint32_t k = 0; k < chunk; ++k) {
+      pef::launch_df_kappa(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+      pef::launch_df_reduce(w.stream, w.N, D.ERR, D.part, D.ctl, D.count, tol, min_it, max_it);
+      pef::launch_df_update(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+    }
+    HIPTRY(hipGetLastError());
+    queued += chunk;
+    HIPTRY(hipMemcpyAsync(&h, D.ctl, sizeof(h), hipMemcpyDeviceToHost, w.stream));
+    HIPTRY(hipStreamSynchronize(w.stream));
+    if (h.done || h.stop) break;
+    chunk = std::min(std::max(chunk * 2u, 1u), max_it - queued);
+  }
+  *iters = h.iter;
+  return PEF_OK;
+}
+
+int32_t dfsph_step(World& w, double dt, uint32_t nsteps) {
+  if (!w.df) return PEF_ERR_UNSUPPORTED;
+  if (!(std::isfinite(dt) && dt > 0.0)) return PEF_ERR_ARG;
+  if (nsteps == 0) return PEF_OK;
+  dt = std::max(dt, 1e-12);  // dfsph.rs:715
+  int32_t rc = df_ensure(w);
+  if (rc) return rc;
+  w.probe_valid = false;
+  pef::DfsphHost& D = *w.df;
+  const pef::DfSlip slip = df_slip(D.prm);
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    pef::launch_df_predict(w.stream, w.N, w.sc_d, w.cur, (float)dt);
+    launch_xsph(w);  // v* + XSPH into B2 (dfsph.rs:381-417)
+    rc = df_solve(w, dt, false, &D.last_density);
+    if (rc) return rc;
+    pef::launch_df_drift(w.stream, w.N, w.sc_d, w.cur, w.ctl_d, dt, slip);
+    enqueue_rebuild(w, nullptr);
+    launch_density(w);
+    launch_alpha(w);
+    rc = df_solve(w, dt, true, &D.last_divergence);
+    if (rc) return rc;
+    pef::launch_df_slip(w.stream, w.N, w.sc_d, w.cur, w.ctl_d, slip);
+    pef::launch_df_finish(w.stream, w.N, w.cur);
+    HIPTRY(hipGetLastError());
+    w.t += dt;
+  }
+  return read_flags(w);
+}
+
+int32_t dfsph_phase(World& w, int32_t phase) {
+  if (!w.df) return PEF_ERR_UNSUPPORTED;
+  w.probe_valid = false;
+  switch (phase) {
+    case PEF_DFSPH_PHASE_REBUILD: {
+      int32_t rc = set_force_rebuild(w);
+      if (rc) return rc;
+      enqueue_rebuild(w, nullptr);
+      break;
+    }
+    case PEF_DFSPH_PHASE_DENSITY:
+      if (!w.density_valid) {
+        int32_t rc = set_force_rebuild(w);
+        if (rc) return rc;
+        enqueue_rebuild(w, nullptr);
+      }
+      launch_density(w);
+      break;
+    case PEF_DFSPH_PHASE_ALPHA:
+      launch_alpha(w);
+      w.density_valid = true;
+      break;
+    default:
+      return PEF_ERR_ARG;
+  }
+  HIPTRY(hipGetLastError());
+  return read_flags(w);
+}
+
+int32_t create_dfsph(const PefDfsphParams* dp, const PefParticles* parts, const PefOptions* opt, PefWorld** out) {
+  if (!dp || !parts || !out) return PEF_ERR_NULL;
+  *out = nullptr;
+  int32_t rc = check_dfsph_params(*dp);
+  if (rc) return rc;
+  if (parts->n > 0 && (!parts->x || !parts->kind)) return PEF_ERR_NULL;
+  PefParams p{};
+  p.dim = dp->dim;
+  p.kernel = dp->kernel;
+  p.rest_density = dp->rest_density;
+  p.mass = dp->mass;
+  p.h = dp->h;
+  p.gamma = 7.0;
+  p.sound_speed = 1.0;  // no equation of state in DFSPH
+  p.xsph_epsilon = dp->xsph_epsilon;
+  for (int a = 0; a < 3; ++a) p.gravity[a] = dp->gravity[a];
+  p.adapt_sound_speed = 0;
+  PefOptions o{};
+  if (opt) o = *opt;
+  o.boundary_force = -1;  // no boundary lists: every DFSPH pass walks fluid particles only
+  // DfsphWorld::new keeps 2-D particles planar (dfsph.rs:148-153)
+  std::vector<double> x, v;
+  PefParticles q = *parts;
+  if (dp->dim == 2 && parts->n > 0) {
+    x.assign(parts->x, parts->x + 3 * parts->n);
+    for (uint64_t i = 0; i < parts->n; ++i) x[3 * i + 2] = 0.0;
+    q.x = x.data();
+    if (parts->v) {
+      v.assign(parts->v, parts->v + 3 * parts->n);
+      for (uint64_t i = 0; i < parts->n; ++i) v[3 * i + 2] = 0.0;
+      q.v = v.data();
+    }
+  }
+  rc = create_world(1, &p, &q, &o, out);
+  if (rc) return rc;
+  World& w = (*out)->w;
+  auto fail = [&](int32_t r) {
+    delete *out;
+    *out = nullptr;
+    return r;
+  };
+  w.df = new (std::nothrow) pef::DfsphHost();
+  if (!w.df) return fail(PEF_ERR_PANIC);
+  pef::DfsphHost& D = *w.df;
+  D.prm = *dp;
+  if (D.prm.particle_diameter <= 0.0) D.prm.particle_diameter = dp->h;
+  if (pef::df_configure(w.tc, D.tiles) != 0) return fail(PEF_ERR_DEVICE);
+  const uint32_t N = w.N;
+  hipError_t e = w.dalloc(&w.cur.DW, (size_t)N + 1);
+  if (e == hipSuccess) e = w.dalloc(&w.alt.DW, (size_t)N + 1);
+  if (e == hipSuccess) e = w.dalloc(&D.AL, (size_t)N + 1);
+  if (e == hipSuccess) e = w.dalloc(&D.ERR, (size_t)N);
+  if (e == hipSuccess) e = w.dalloc(&D.part, (size_t)pef::df_reduce_blocks(N));
+  if (e == hipSuccess) e = w.dalloc(&D.ctl, 1);
+  if (e == hipSuccess) e = hipMemsetAsync(w.cur.DW, 0, ((size_t)N + 1) * sizeof(float2), w.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(D.AL, 0, ((size_t)N + 1) * sizeof(float), w.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(D.ERR, 0, (size_t)N * sizeof(double), w.stream);
+  if (e != hipSuccess) return fail(PEF_ERR_PANIC);
+  uint64_t nf = 0;
+  for (uint64_t i = 0; i < parts->n; ++i) nf += parts->kind[i] == 0 ? 1u : 0u;
+  D.count = (double)nf;
+  const double hs = std::max(dp->h, 1e-12);
+  const double cg = dp->dim == 2 ? dp->mass / (hs * hs * hs) : dp->mass / (hs * hs * hs * hs);
+  D.floor_ = (float)std::max(1e-6 * cg * cg, 1e-12);
+  w.scenes[0].dfsph = 1;
+  e = hipMemcpyAsync(w.sc_d, w.scenes.data(), sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream);
+  if (e != hipSuccess) return fail(PEF_ERR_PANIC);
+  w.density_valid = false;
+  rc = df_ensure(w);
+  if (!rc) rc = read_flags(w);
+  if (rc) return fail(rc);
   return PEF_OK;
 }
 
@@ -1577,12 +1833,73 @@ void pef_world_destroy(PefWorld* world) {
 
 int32_t pef_world_step(PefWorld* world, double dt, uint32_t nsteps) {
   if (!world) return PEF_ERR_NULL;
+  if (world->w.df) return PEF_ERR_UNSUPPORTED;  // a DFSPH world steps with pef_dfsph_step
   return guard([&] { return do_step(world->w, dt, nsteps); });
 }
 
 int32_t pef_world_phase(PefWorld* world, int32_t phase) {
   if (!world) return PEF_ERR_NULL;
+  if (world->w.df) return PEF_ERR_UNSUPPORTED;
   return guard([&] { return do_phase(world->w, phase); });
+}
+
+int32_t pef_dfsph_default_params(int32_t dim, double rest_density, double mass, double h, PefDfsphParams* out) {
+  if (!out) return PEF_ERR_NULL;
+  PefDfsphParams p{};
+  p.dim = dim;
+  p.kernel = PEF_KERNEL_CUBIC_SPLINE;
+  p.rest_density = rest_density;
+  p.mass = mass;
+  p.h = h;
+  p.max_density_iters = 50;
+  p.max_divergence_iters = 50;
+  p.density_error_tol = 0.001;
+  p.divergence_error_tol = 0.001;
+  p.gravity[2] = -9.80665;
+  p.xsph_epsilon = 0.05;
+  p.cfl_lambda = 0.4;
+  p.adaptive_dt = 1;
+  p.warm_start = 1;
+  p.solver_omega = 1.0;
+  *out = p;
+  return PEF_OK;
+}
+
+int32_t pef_dfsph_create(const PefDfsphParams* params, const PefParticles* particles, const PefOptions* options,
+                         PefWorld** out) {
+  return guard([&] { return create_dfsph(params, particles, options, out); });
+}
+
+int32_t pef_dfsph_step(PefWorld* world, double dt, uint32_t nsteps) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return dfsph_step(world->w, dt, nsteps); });
+}
+
+int32_t pef_dfsph_phase(PefWorld* world, int32_t phase) {
+  if (!world) return PEF_ERR_NULL;
+  return guard([&] { return dfsph_phase(world->w, phase); });
+}
+
+int32_t pef_dfsph_iterations(PefWorld* world, uint32_t* density_iters, uint32_t* divergence_iters) {
+  if (!world || !density_iters || !divergence_iters) return PEF_ERR_NULL;
+  if (!world->w.df) return PEF_ERR_UNSUPPORTED;
+  *density_iters = world->w.df->last_density;
+  *divergence_iters = world->w.df->last_divergence;
+  return PEF_OK;
+}
+
+int32_t pef_dfsph_get_alpha(PefWorld* world, double* out, uint64_t len) {
+  if (!world || !out) return PEF_ERR_NULL;
+  World& w = world->w;
+  if (!w.df) return PEF_ERR_UNSUPPORTED;
+  if (len < w.N) return PEF_ERR_BOUNDS;
+  return guard([&] {
+    pef::launch_df_unsort_f32(w.stream, w.N, w.cur, w.df->AL, w.stage);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(out, w.stage, (size_t)w.N * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    HIPTRY(hipStreamSynchronize(w.stream));
+    return (int32_t)PEF_OK;
+  });
 }
 
 int32_t pef_world_len(const PefWorld* world, uint64_t* n) {
@@ -1628,6 +1945,7 @@ int32_t pef_world_get_params(PefWorld* world, uint32_t scene, PefParams* out) {
 int32_t pef_world_set_params(PefWorld* world, uint32_t scene, const PefParams* params) {
   if (!world || !params) return PEF_ERR_NULL;
   World& w = world->w;
+  if (w.df) return PEF_ERR_UNSUPPORTED;  // WCSPH parameters
   if (scene >= w.S) return PEF_ERR_BOUNDS;
   int32_t rc = check_params(*params);
   if (rc) return rc;
@@ -1718,6 +2036,7 @@ int32_t pef_world_sample(PefWorld* world, PefDataset* ds, uint64_t n, uint64_t s
 int32_t pef_world_set_surrogate(PefWorld* world, PefMlp* mlp, uint32_t check_every, double accel_tol,
                                 double density_tol) {
   if (!world) return PEF_ERR_NULL;
+  if (world->w.df) return PEF_ERR_UNSUPPORTED;
   return guard([&] { return set_surrogate(world->w, mlp, check_every, accel_tol, density_tol); });
 }
 
@@ -1785,6 +2104,7 @@ int32_t pef_world_front_x_height(PefWorld* world, uint32_t scene, double dx, dou
 int32_t pef_world_autotune(PefWorld* world, double dt, uint32_t steps, PefTuneEntry* table, uint32_t cap,
                            uint32_t* n, PefTuneEntry* best) {
   if (!world) return PEF_ERR_NULL;
+  if (world->w.df) return PEF_ERR_UNSUPPORTED;
   return guard([&] { return autotune(world->w, dt, steps, table, cap, n, best); });
 }
 

@@ -115,6 +115,31 @@ class PefDiagnostics(C.Structure):
 _lib = None
 
 
+class PefDfsphParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("kernel", C.c_int32),
+        ("rest_density", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("max_density_iters", C.c_uint32),
+        ("max_divergence_iters", C.c_uint32),
+        ("density_error_tol", C.c_double),
+        ("divergence_error_tol", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("xsph_epsilon", C.c_double),
+        ("surface_tension_sigma", C.c_double),
+        ("particle_diameter", C.c_double),
+        ("cfl_lambda", C.c_double),
+        ("adaptive_dt", C.c_int32),
+        ("warm_start", C.c_int32),
+        ("free_slip_on", C.c_int32 * 4),
+        ("free_slip_at", C.c_double * 4),
+        ("solver_omega", C.c_double),
+        ("max_velocity_correction", C.c_double),
+    ]
+
+
 class PefSurrogateStats(C.Structure):
     _fields_ = [("surrogate_steps", C.c_uint64), ("plain_steps", C.c_uint64), ("checks", C.c_uint64),
                 ("failed_checks", C.c_uint64), ("last_accel_error", C.c_double),
@@ -191,6 +216,12 @@ def lib():
         "pef_world_sync_rigid": (i32, [vp, u32, vp, vp, vp, vp]),
         "pef_world_reduce_rigid": (i32, [vp, u32, vp, vp, vp]),
         "pef_world_surrogate_stats": (i32, [vp, P(PefSurrogateStats)]),
+        "pef_dfsph_default_params": (i32, [i32, d, d, d, P(PefDfsphParams)]),
+        "pef_dfsph_create": (i32, [P(PefDfsphParams), P(PefParticles), vp, P(vp)]),
+        "pef_dfsph_step": (i32, [vp, d, u32]),
+        "pef_dfsph_phase": (i32, [vp, i32]),
+        "pef_dfsph_iterations": (i32, [vp, P(u32), P(u32)]),
+        "pef_dfsph_get_alpha": (i32, [vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -780,3 +811,85 @@ def version() -> str:
 
 
 from . import scenes  # noqa: E402,F401
+
+
+# --- dfsph.rs ------------------------------------------------------------------
+@dataclass
+class DfsphParams:
+    """DfsphParams (dfsph.rs:33-98); the defaults are DfsphParams::default()."""
+
+    max_density_iters: int = 50
+    max_divergence_iters: int = 50
+    density_error_tol: float = 0.001
+    divergence_error_tol: float = 0.001
+    gravity: Sequence[float] = (0.0, 0.0, -9.80665)
+    xsph_epsilon: float = 0.05
+    surface_tension_sigma: float = 0.0
+    particle_diameter: float = 0.0
+    cfl_lambda: float = 0.4
+    adaptive_dt: bool = True
+    warm_start: bool = True
+    free_slip_xmin: Optional[float] = None
+    free_slip_ymin: Optional[float] = None
+    free_slip_xmax: Optional[float] = None
+    free_slip_ymax: Optional[float] = None
+    solver_omega: float = 1.0
+    max_velocity_correction: float = 0.0
+
+    def to_c(self, parts: Particles) -> PefDfsphParams:
+        p = PefDfsphParams()
+        _check(lib().pef_dfsph_default_params(parts.dim, parts.rest_density, parts.mass, parts.h, C.byref(p)),
+               "pef_dfsph_default_params")
+        p.max_density_iters, p.max_divergence_iters = self.max_density_iters, self.max_divergence_iters
+        p.density_error_tol, p.divergence_error_tol = self.density_error_tol, self.divergence_error_tol
+        p.gravity[:] = [float(g) for g in self.gravity]
+        p.xsph_epsilon, p.surface_tension_sigma = self.xsph_epsilon, self.surface_tension_sigma
+        p.particle_diameter, p.cfl_lambda = self.particle_diameter, self.cfl_lambda
+        p.adaptive_dt, p.warm_start = int(bool(self.adaptive_dt)), int(bool(self.warm_start))
+        for k, b in enumerate((self.free_slip_xmin, self.free_slip_ymin, self.free_slip_xmax, self.free_slip_ymax)):
+            p.free_slip_on[k] = 0 if b is None else 1
+            p.free_slip_at[k] = 0.0 if b is None else float(b)
+        p.solver_omega, p.max_velocity_correction = self.solver_omega, self.max_velocity_correction
+        return p
+
+
+class DfsphWorld(WcsphWorld):
+    """DfsphWorld (dfsph.rs:100-800) backed by the device: the WCSPH world's
+    grid, lists, density and XSPH passes with the DFSPH solvers. Getters are
+    the WcsphWorld ones (pressures read kappa)."""
+
+    def __init__(self, particles: Particles, params: DfsphParams, options: Optional[Options] = None):
+        self._h = C.c_void_p()
+        keep: list = []
+        cp = _c_particles(particles, keep)
+        pp = params.to_c(particles)
+        opts = (options or Options()).to_c()
+        _check(lib().pef_dfsph_create(C.byref(pp), C.byref(cp), C.byref(opts), C.byref(self._h)), "pef_dfsph_create")
+        self.scenes, self._params = [particles], [params]
+        n = C.c_uint64()
+        _check(lib().pef_world_len(self._h, C.byref(n)), "pef_world_len")
+        self.n = n.value
+        self.dim = particles.dim
+        self.params = params
+
+    def step(self, dt: float, nsteps: int = 1):
+        _check(lib().pef_dfsph_step(self._h, dt, nsteps), "pef_dfsph_step")
+
+    def rebuild_neighbours(self):
+        _check(lib().pef_dfsph_phase(self._h, 0), "rebuild_neighbours")
+
+    def compute_density(self):
+        _check(lib().pef_dfsph_phase(self._h, 1), "compute_density")
+
+    def compute_alpha(self):
+        _check(lib().pef_dfsph_phase(self._h, 2), "compute_alpha")
+
+    def iterations(self):
+        """(density solver, divergence solver) iterations of the last step."""
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(lib().pef_dfsph_iterations(self._h, C.byref(a), C.byref(b)), "pef_dfsph_iterations")
+        return a.value, b.value
+
+    def alpha(self) -> np.ndarray:
+        return self._get("pef_dfsph_get_alpha", False)
+
