@@ -102,6 +102,8 @@ struct Ctl {
   uint32_t flags;          // sticky kFlag* bits
   uint32_t max_count;      // largest list length seen at the last build
   uint32_t slow_bricks;    // bricks whose halo exceeded the LDS capacity at the last build
+  uint32_t band_count;     // one-sided pairs near their visibility plane at the last build
+  uint32_t band_overflow;  // more than the band holds: rebuild at every step
   unsigned long long list_entries;  // total list entries at the last build
   unsigned long long rebuilds;
 };

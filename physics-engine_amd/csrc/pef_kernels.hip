@@ -29,6 +29,8 @@ __global__ void k_decide(Ctl* ctl, SceneState* st, uint32_t n_scenes) {
     if (r) {
       ctl->rebuilds += 1ull;
       ctl->slow_bricks = 0u;
+      ctl->band_count = 0u;  // the list build records the band afresh
+      ctl->band_overflow = 0u;
     }
   }
   (void)st;
@@ -646,6 +648,7 @@ __global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const 
     vs[2] = 0.0;
   }
   const double ox = cur.X[s], oy = cur.Y[s], oz = cur.Z[s];
+  const double onx = cur.NX[s], ony = cur.NY[s], onz = cur.NZ[s];
   cur.X[s] = p[0];
   cur.Y[s] = p[1];
   cur.Z[s] = p[2];
@@ -663,7 +666,14 @@ __global__ void k_rigid_sync(uint32_t n, const uint32_t* __restrict__ id, const 
   cur.B[s] = make_float4((float)vs[0], (float)vs[1], (float)vs[2], cur.B[s].w);
   cur.B2[s] = make_float4((float)vs[0], (float)vs[1], (float)vs[2], cur.B2[s].w);
   const double dx = p[0] - ox, dy = p[1] - oy, dz = p[2] - oz;
-  const float d = cur.DISP[s] + (float)sqrt(dx * dx + dy * dy + dz * dz);
+  // a turned normal moves the visibility plane of every partner within the
+  // list radius by at most radius |dn| / |n|: it counts as displacement
+  const double n2 = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+  const bool oriented = (cur.META[s] & kMetaOriented) != 0u;  // only those normals are kept
+  const double tn = (oriented && n2 > 0.0) ? S.cell * sqrt((nn[0] - onx) * (nn[0] - onx) + (nn[1] - ony) * (nn[1] - ony) +
+                                            (nn[2] - onz) * (nn[2] - onz)) / sqrt(n2)
+                             : 0.0;
+  const float d = cur.DISP[s] + (float)(sqrt(dx * dx + dy * dy + dz * dz) + tn);
   cur.DISP[s] = d;
   if (!(d <= S.skin_half)) ctlw->need_rebuild = 1u;
 }

@@ -30,6 +30,8 @@ struct TileArgs {
   uint32_t cap;      // list entries per particle (multiple of 4)
   uint32_t lds_cap;  // halo particles the kernel stages into LDS (set per kernel by the launcher)
   uint32_t N;        // sentinel slot
+  uint4* band;        // list build: one-sided pairs near their visibility plane (fluid slot, sample slot, visible)
+  uint32_t band_cap;
 };
 
 // Launch configuration, filled by tiles_configure from the device limits.
@@ -53,6 +55,9 @@ void launch_brick_list(hipStream_t st, uint32_t NB, const Ctl* ctl, int always, 
                        const uint32_t* META, void* tables);
 void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const Ctl* ctl, Ctl* ctlw,
                         const SceneDev* sc, const State& cur, int with_boundary, int always);
+// Rebuild at the next decision when a recorded one-sided pair changed sides.
+void launch_band_check(hipStream_t st, const uint4* band, uint32_t band_cap, const SceneDev* sc, const State& cur,
+                       Ctl* ctlw);
 void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
                           const SceneState* ss, const State& cur);
 void launch_xsph_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,

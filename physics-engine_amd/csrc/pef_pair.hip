@@ -395,6 +395,42 @@ __device__ inline bool sees_f64(double bx, double by, double bz, double nx, doub
   return d > -eps;
 }
 
+// The list keeps a one-sided pair's visibility from its build. Until the next
+// build each particle moves less than skin/2 and a sample's normal turns less
+// than that over the list radius (k_rigid_sync counts it as displacement), so
+// (x_f - x_b).n moves less than skin |n|: a pair farther than that from its
+// plane keeps its side. Nearer pairs go into the band, which
+// k_band_check re-tests before every rebuild decision.
+__device__ inline void band_record(const TileArgs& ta, Ctl* ctlw, const SceneDev& S, const State& cur, uint32_t f,
+                                   uint32_t b, bool vis) {
+#pragma clang fp contract(off)
+  const double nx = cur.NX[b], ny = cur.NY[b], nz = cur.NZ[b];
+  const double n = sqrt(nx * nx + ny * ny + nz * nz);
+  const double v = (cur.X[f] - cur.X[b]) * nx + (cur.Y[f] - cur.Y[b]) * ny + (cur.Z[f] - cur.Z[b]) * nz +
+                   0.15 * S.h_d * n;
+  if (!(fabs(v) <= 2.0 * (double)S.skin_half * n * 1.0001)) return;
+  const uint32_t k = atomicAdd(&ctlw->band_count, 1u);
+  if (k < ta.band_cap) ta.band[k] = make_uint4(f, b, vis ? 1u : 0u, 0u);
+  else ctlw->band_overflow = 1u;
+}
+
+__global__ void k_band_check(const uint4* __restrict__ band, uint32_t band_cap, const SceneDev* __restrict__ sc,
+                             State cur, Ctl* ctlw) {
+  if (ctlw->band_overflow) {  // the band did not hold every pair: rebuild at every step
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctlw->need_rebuild = 1u;
+    return;
+  }
+  const uint32_t n = ctlw->band_count < band_cap ? ctlw->band_count : band_cap;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint4 e = band[k];
+    const uint32_t f = e.x, b = e.y;
+    const SceneDev& S = sc[scene_of(cur.META[b])];
+    const bool vis = sees_f64(cur.X[b], cur.Y[b], cur.Z[b], cur.NX[b], cur.NY[b], cur.NZ[b], cur.X[f], cur.Y[f],
+                              cur.Z[f], S.h_d);
+    if (vis != (e.z != 0u)) ctlw->need_rebuild = 1u;
+  }
+}
+
 // Slot of halo-local index t inside the three-cell z-row starting at cell k0.
 __device__ inline uint32_t row_slot(const HaloTab& T, uint32_t k0, uint32_t t) {
   const uint32_t k = k0 + (t >= T.pre[k0 + 1] ? 1u : 0u) + (t >= T.pre[k0 + 2] ? 1u : 0u);
@@ -410,7 +446,8 @@ constexpr uint64_t kNlistGroupMask = (1ull << kNlistLanes) - 1ull;
 
 template <bool FAST>
 __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, const Own& o, uint32_t sub,
-                                       const float4* LA, State& cur, uint16_t* LIST, uint32_t cap) {
+                                       const float4* LA, State& cur, uint16_t* LIST, uint32_t cap,
+                                       const TileArgs& ta, Ctl* ctlw) {
   const uint32_t s = o.s;
   const uint32_t meta = cur.META[s];
   const bool i_bnd = (meta & kMetaBoundary) != 0u;
@@ -442,9 +479,11 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
             if (!i_bnd) {
               if (j_bnd && S.oriented) {
                 const uint32_t j = row_slot(T, k0, t);
-                if (cur.META[j] & kMetaOriented)
+                if (cur.META[j] & kMetaOriented) {
                   ok = sees_f64(cur.X[j], cur.Y[j], cur.Z[j], cur.NX[j], cur.NY[j], cur.NZ[j], cur.X[s], cur.Y[s],
                                 cur.Z[s], S.h_d);
+                  band_record(ta, ctlw, S, cur, s, j, ok);
+                }
               }
             } else {
               ok = !j_bnd;
@@ -452,6 +491,7 @@ __device__ inline uint32_t nlist_group(const HaloTab& T, const SceneDev& S, cons
                 const uint32_t j = row_slot(T, k0, t);
                 ok = sees_f64(cur.X[s], cur.Y[s], cur.Z[s], cur.NX[s], cur.NY[s], cur.NZ[s], cur.X[j], cur.Y[j],
                               cur.Z[j], S.h_d);
+                band_record(ta, ctlw, S, cur, j, s, ok);
               }
             }
           }
@@ -501,8 +541,8 @@ __global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __
         uint32_t cnt = 0;
         const bool i_bnd = (cur.META[o.s] & kMetaBoundary) != 0u;
         if (encodable && (!i_bnd || with_boundary))
-          cnt = fast ? nlist_group<true>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap)
-                     : nlist_group<false>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap);
+          cnt = fast ? nlist_group<true>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap, ta, ctlw)
+                     : nlist_group<false>(Tb, S, o, sub, LA, cur, ta.LIST, ta.cap, ta, ctlw);
         if (sub == 0u) ta.CNT[o.s] = cnt < ta.cap ? cnt : ta.cap;
         mymax = mymax > cnt ? mymax : cnt;
       }
@@ -1491,6 +1531,11 @@ void launch_nlist_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta,
       default: hipLaunchKernelGGL((K<16, 1024>), g_, b_, bytes, st, __VA_ARGS__); break;           \
     }                                                                                               \
   } while (0)
+
+void launch_band_check(hipStream_t st, const uint4* band, uint32_t band_cap, const SceneDev* sc, const State& cur,
+                       Ctl* ctlw) {
+  PEF_LAUNCH_LDS(k_band_check, 64, 256, 0, st, band, band_cap, sc, cur, ctlw);
+}
 
 void launch_density_tiled(hipStream_t st, const TileConfig& c, const TileArgs& ta, const SceneDev* sc,
                           const SceneState* ss, const State& cur) {

@@ -105,6 +105,9 @@ class World {
   void* rs_mem = nullptr;
   // DFSPH solver state (pef_dfsph_create); null for a WCSPH world
   struct DfsphHost* df = nullptr;
+  // one-sided pairs near their visibility plane at the last list build
+  uint4* band = nullptr;
+  uint32_t band_cap = 0;
 
   ~World() { release(); }
 
@@ -117,6 +120,9 @@ class World {
   }
   void release();
   void release_buffers() {
+    if (band) (void)hipFree(band);
+    band = nullptr;
+    band_cap = 0;
     if (rs_mem) (void)hipFree(rs_mem);
     rs_mem = nullptr;
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -235,6 +241,8 @@ void fill_scene_physics(pef::SceneDev& S, const PefParams& p, double skin) {
   S.rho0_d = p.rest_density;
   S.gamma_d = p.gamma;
 }
+
+int32_t ensure_band(World& w);
 
 int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefParticles* parts, const PefOptions* opt,
                      PefWorld** out) {
@@ -503,6 +511,7 @@ int32_t create_world(uint32_t n_scenes, const PefParams* params, const PefPartic
   w.h_psi = std::move(hpsi);
   w.h_nout = std::move(hn);
   w.h_kind = std::move(hk);
+  if (ensure_band(w) != PEF_OK) return fail(PEF_ERR_PANIC);
   *out = pw;
   return PEF_OK;
 }
@@ -519,7 +528,34 @@ pef::TileArgs tile_args(World& w) {
   a.cap = w.cap;
   a.lds_cap = 0;
   a.N = w.N;
+  a.band = w.band;
+  a.band_cap = w.band_cap;
   return a;
+}
+
+// The band holds the one-sided pairs near their visibility plane; about eight
+// per sample and list direction are expected, 64 per sample fit. A larger
+// need reallocates it, which invalidates the captured step.
+int32_t ensure_band(World& w) {
+  uint64_t n_or = 0;
+  for (uint64_t i = 0; i < w.h_kind.size(); ++i) {
+    const double* n = &w.h_nout[3 * i];
+    if (w.h_kind[i] && n[0] * n[0] + n[1] * n[1] + n[2] * n[2] > 0.0) ++n_or;
+  }
+  if (n_or == 0) return PEF_OK;
+  const uint64_t need = std::min<uint64_t>(std::max<uint64_t>(1024, 64 * n_or), 1u << 30);
+  if (need <= w.band_cap) return PEF_OK;
+  if (w.graph) {
+    (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+  }
+  HIPTRY(hipStreamSynchronize(w.stream));
+  if (w.band) HIPTRY(hipFree(w.band));
+  w.band = nullptr;
+  w.band_cap = 0;
+  HIPTRY(hipMalloc(&w.band, need * sizeof(uint4)));
+  w.band_cap = (uint32_t)need;
+  return PEF_OK;
 }
 
 void launch_alpha(World& w);
@@ -587,6 +623,7 @@ struct Prof {
 };
 
 void enqueue_rebuild(World& w, Prof* pr) {
+  if (w.band_cap) pef::launch_band_check(w.stream, w.band, w.band_cap, w.sc_d, w.cur, w.ctl_d);
   pef::launch_decide(w.stream, w.ctl_d, w.st_d, w.S);
   if (pr) pr->mark("decide");
   pef::launch_grid_build(w.stream, grid_args(w), 0);
@@ -1619,6 +1656,8 @@ int32_t set_state(World& w, const double* x, const double* v, const double* psi,
   if (e != hipSuccess) return PEF_ERR_PANIC;
   w.density_valid = false;
   w.probe_valid = false;
+  int32_t rb = ensure_band(w);  // new normals may add one-sided samples
+  if (rb) return rb;
   return set_force_rebuild(w);
 }
 

@@ -181,3 +181,44 @@ def test_rigid_sample_errors():
         W.sync_rigid(np.zeros((2, 3)), np.tile([1.0, 0, 0, 0], (2, 1)))  # body count differs
     with pytest.raises(pef.PefError):
         W.sync_rigid(np.array([[np.nan, 0, 0]]), np.array([[1.0, 0, 0, 0]]))
+
+
+def test_one_sided_visibility_follows_particles_between_list_builds():
+    """A fluid block moving at 0.01 h per step toward the back of a one-sided
+    plate (samples at y = 0, normals +y, boundary_sees_fluid: y_f > -0.15 h):
+    its top layer crosses the visibility plane after 8 steps, between the list
+    builds the skin asks for after steps 5 and 10. Densities follow the oracle,
+    which evaluates visibility for every pair at every step."""
+    dx, h = 0.01, 0.02
+    m = O.mass_calibrated_to_rest(1000.0, dx, h, 3)
+    xs = np.stack(np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij"), -1).reshape(-1, 3) * dx
+    fluid = xs + np.array([0.005, -0.0745, 0.005])  # top layer at y = -0.0045
+    gx, gz = np.meshgrid(np.arange(-2, 11), np.arange(-2, 11), indexing="ij")
+    plate = np.stack([gx.reshape(-1) * dx, np.zeros(gx.size), gz.reshape(-1) * dx], 1)
+    x = np.concatenate([fluid, plate])
+    kind = np.concatenate([np.zeros(len(fluid), np.uint8), np.ones(len(plate), np.uint8)])
+    v = np.zeros_like(x)
+    v[: len(fluid), 1] = 2.0
+    psi = np.concatenate([np.zeros(len(fluid)), np.full(len(plate), dx ** 3)])
+    n_out = np.zeros_like(x)
+    n_out[len(fluid):, 1] = 1.0
+    G = pef.Particles(3, 1000.0, m, h)
+    G._append(x, v, kind, psi, n_out)
+    params = pef.WcsphParams.water(1000.0, 40.0, (0.0, 0.0, 0.0))
+    params.visc_alpha = 0.0
+    params.xsph_epsilon = 0.0
+    params.adapt_sound_speed = False
+    W = pef.WcsphWorld(G, params)
+    prm = dict(dim=3, rest_density=1000.0, mass=m, h=h, sound_speed=40.0, visc_alpha=0.0, xsph_eps=0.0,
+               gravity=(0.0, 0.0, 0.0), adapt_sound_speed=False)
+    Wo = O.World(prm, x, kind, v=v, psi=psi, n_out=n_out, threads=4)
+    fl = kind == 0
+    dt = 1e-4
+    crossed = False
+    for k in range(14):
+        W.step(dt)
+        Wo.step(dt)
+        rho, rho_o = W.densities()[fl], Wo.state()["rho"][fl]
+        assert np.abs(rho / rho_o - 1).max() <= 1e-4, (k, np.abs(rho / rho_o - 1).max())
+        crossed = crossed or (Wo.state()["x"].reshape(-1, 3)[fl, 1] > -0.15 * h).any()
+    assert crossed
