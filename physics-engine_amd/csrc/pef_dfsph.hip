@@ -73,8 +73,8 @@ __global__ void k_df_warm_zero(uint32_t N, State cur, int which) {
 }
 
 // Mean of ERR over the fluid particles, FP64 in a fixed order: each workgroup
-// sums a fixed slot range by a fixed tree, the last one to finish adds the
-// workgroup sums in index order. Then the stopping rule: avg <= tol after at
+// sums a fixed slot range by a fixed tree, and the last one to finish sums the
+// workgroup sums by the same tree. Then the stopping rule: avg <= tol after at
 // least min_iter iterations, or max_iter iterations (dfsph.rs:573-577,
 // 616-620).
 constexpr uint32_t kDfReduceThreads = 256;
@@ -95,14 +95,26 @@ __global__ void __launch_bounds__(kDfReduceThreads) k_df_reduce(uint32_t N, cons
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
+  __shared__ uint32_t last;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0];
+    __threadfence();
+    last = atomicAdd(&c->arrive, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __threadfence();
+  }
+  __syncthreads();
+  if (!last) return;
+  // the last workgroup: the workgroup sums by the same fixed strided tree
+  acc = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kDfReduceThreads) acc += part[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t o = kDfReduceThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
   if (threadIdx.x != 0) return;
-  part[blockIdx.x] = red[0];
-  __threadfence();
-  if (atomicAdd(&c->arrive, 1u) != gridDim.x - 1u) return;
-  __threadfence();
-  double tot = 0.0;
-  for (uint32_t b = 0; b < gridDim.x; ++b) tot += part[b];  // the fences order these after every block's write
-  const double avg = count > 0.0 ? tot / count : 0.0;
+  const double avg = count > 0.0 ? red[0] / count : 0.0;
   const uint32_t it = c->iter + 1u;
   c->iter = it;
   c->avg = avg;

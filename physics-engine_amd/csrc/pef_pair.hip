@@ -1042,7 +1042,7 @@ __global__ void k_sur_check(uint32_t N, const SceneDev* __restrict__ sc, State c
 // the build); the sentinel weighs 0. Only fluid particles are evaluated.
 // ---------------------------------------------------------------------------
 constexpr int kDfLpp = 4;
-constexpr int kDfThreads = 512;
+constexpr int kDfThreads = 1024;
 
 // alpha_i = rho_i / max(|sum m_j grad W|^2 + sum |m_j grad W|^2, floor), 0 with
 // fewer than two candidates (dfsph.rs:247-299). The reference counts the
@@ -1585,7 +1585,7 @@ void launch_sur_check(hipStream_t st, uint32_t N, const SceneDev* sc, const Stat
   PEF_LAUNCH_LDS(k_sur_check, blocks_of(N, 256), 256, 0, st, N, sc, cur, pred, part);
 }
 
-// DFSPH passes: one instantiation (four lanes per particle, 512 threads); the
+// DFSPH passes: one instantiation (four lanes per particle, 1 024 threads); the
 // one-array alpha pass uses the density pass's LDS capacity, the others the
 // XSPH pass's.
 int df_configure(const TileConfig& c, DfTiles& d) {
