@@ -308,7 +308,7 @@ def dfsph_leg(cfg, steps, dt, ms_wcsph):
     prm = pef.DfsphParams(gravity=tuple(cfg.gravity), particle_diameter=cfg.dx, density_error_tol=0.01,
                           divergence_error_tol=0.01, max_density_iters=40, max_divergence_iters=25,
                           xsph_epsilon=0.15, max_velocity_correction=0.8, warm_start=False)
-    D = pef.DfsphWorld(parts, prm)
+    D = pef.DfsphWorld(parts, prm, pef.Options(boundary_force=False))  # no body to drive: no reactions
     D.step(dt, 5)
     its = []
     t0 = time.perf_counter()

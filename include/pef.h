@@ -279,8 +279,9 @@ int32_t pef_dfsph_create(const PefDfsphParams* params, const PefParticles* parti
  * velocities, XSPH on them, the constant-density solver, drift and free-slip
  * planes, lists and densities and alpha of the new positions, the
  * divergence-free solver, free-slip planes, v = v*. Synchronous. Boundary
- * reactions (dfsph.rs:490-522) are not computed: pef_world_get_boundary_force
- * reads zeros. */
+ * reactions (dfsph.rs:490-522) are accumulated over the step for
+ * pef_world_get_boundary_force and pef_world_reduce_rigid unless the options
+ * set boundary_force = -1 (then they read zeros). */
 int32_t pef_dfsph_step(PefWorld* world, double dt, uint32_t nsteps);
 #define PEF_DFSPH_PHASE_REBUILD 0  /* rebuild_neighbours (dfsph.rs:199-209) */
 #define PEF_DFSPH_PHASE_DENSITY 1  /* compute_density (dfsph.rs:212-239) */

@@ -167,6 +167,12 @@ def lib():
         "ora_cw_rigid_set": (None, [vp, C.c_uint32, vp, vp, vp]),
         "ora_cw_wrench": (None, [vp, vp, vp]),
         "ora_cw_time": (d, [vp]),
+        "ora_cdf_new": (vp, [vp, u64, vp, vp, vp, vp, vp]),
+        "ora_cdf_free": (None, [vp]),
+        "ora_cdf_step": (None, [vp, d, u64]),
+        "ora_cdf_rigid_pack": (None, [vp, vp]),
+        "ora_cdf_wrench": (None, [vp, vp, vp]),
+        "ora_cdf_fluid": (vp, [vp]),
         "ora_mlp_forward": (None, [vp, vp, vp, u64, i32, vp]),
         "ora_mlp_grad": (None, [vp, vp, vp, vp, u64, i32, vp, pd]),
         "ora_adam": (None, [vp, vp, vp, vp, u64, d, d, d, d, C.c_uint32]),
@@ -423,6 +429,51 @@ class CoupledWcsph:
 
     def set_visc_xsph(self, visc_alpha, xsph_eps):
         lib().ora_world_set_visc_xsph(self.world.hnd, visc_alpha, xsph_eps)
+
+
+class CoupledDfsph:
+    """CoupledDfsphWorld (coupling.rs:353-408): a copy of an oracle Dfsph world,
+    the coupler's samples and a rigid world."""
+
+    def __init__(self, dfsph: "Dfsph", samples, bodies, gravity):
+        particle, body, local, local_n = (np.ascontiguousarray(a) for a in samples)
+        particle = particle.astype(np.uint64)
+        body = body.astype(np.uint32)
+        local = local.astype(np.float64).reshape(-1)
+        local_n = local_n.astype(np.float64).reshape(-1)
+        rigid = RigidWorld(bodies, gravity)
+        self.n_bodies = rigid.n
+        self.n = dfsph.n
+        self.hnd = lib().ora_cdf_new(dfsph.hnd, len(particle), _p(particle), _p(body), _p(local), _p(local_n),
+                                     rigid.hnd)
+        rigid.hnd = None
+
+    def __del__(self):
+        if getattr(self, "hnd", None):
+            lib().ora_cdf_free(self.hnd)
+
+    def step(self, dt, nsteps=1):
+        lib().ora_cdf_step(self.hnd, dt, nsteps)
+
+    def rigid_state(self):
+        out = np.zeros(13 * self.n_bodies)
+        lib().ora_cdf_rigid_pack(self.hnd, _p(out))
+        nb = self.n_bodies
+        return dict(x=out[:3 * nb].reshape(nb, 3), q=out[3 * nb:7 * nb].reshape(nb, 4),
+                    v=out[7 * nb:10 * nb].reshape(nb, 3), l=out[10 * nb:].reshape(nb, 3))
+
+    def wrench(self):
+        f, t = np.zeros((self.n_bodies, 3)), np.zeros((self.n_bodies, 3))
+        lib().ora_cdf_wrench(self.hnd, _p(f), _p(t))
+        return f, t
+
+    def fluid_state(self):
+        n = self.n
+        out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), rho=np.zeros(n), p=np.zeros(n), alpha=np.zeros(n),
+                   boundary_force=np.zeros((n, 3)))
+        lib().ora_dfsph_get_state(lib().ora_cdf_fluid(self.hnd),
+                                  *(_p(out[k]) for k in ("x", "v", "rho", "p", "alpha", "boundary_force")))
+        return out
 
 
 def fnv1a_f64(vals):

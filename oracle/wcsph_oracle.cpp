@@ -2016,6 +2016,39 @@ void ora_cw_wrench(void* cw, double* force, double* torque) {
   }
 }
 double ora_cw_time(void* cw) { return ((CoupledWcsph*)cw)->R.t; }
+
+// --- coupled DFSPH world (coupling.rs:353-408) over an oracle Dfsph world
+// (copied in) with the coupler's samples and a rigid world (taken over).
+void* ora_cdf_new(void* dfsph, uint64_t ns, const uint64_t* particle, const uint32_t* body, const double* local,
+                  const double* local_n, void* rigid) {
+  CoupledDfsph* c = new CoupledDfsph();
+  c->D = *(Dfsph*)dfsph;
+  RigidWorld* R = (RigidWorld*)rigid;
+  c->R = *R;
+  delete R;
+  c->C.resize_bodies(c->R.n);
+  for (uint64_t k = 0; k < ns; ++k)
+    c->C.samples.push_back(RigidSample{body[k], v3(local[3 * k], local[3 * k + 1], local[3 * k + 2]),
+                                       v3(local_n[3 * k], local_n[3 * k + 1], local_n[3 * k + 2]), particle[k]});
+  return c;
+}
+void ora_cdf_free(void* c) { delete (CoupledDfsph*)c; }
+void ora_cdf_step(void* c, double dt, uint64_t steps) {
+  for (uint64_t k = 0; k < steps; ++k) ((CoupledDfsph*)c)->step(dt);
+}
+void ora_cdf_rigid_pack(void* c, double* out) {
+  const std::vector<double> pk = ((CoupledDfsph*)c)->R.pack();
+  std::memcpy(out, pk.data(), pk.size() * 8);
+}
+void ora_cdf_wrench(void* c, double* force, double* torque) {
+  const Coupler& C = ((CoupledDfsph*)c)->C;
+  for (size_t b = 0; b < C.force.size(); ++b) {
+    const double a[6] = {C.force[b].x, C.force[b].y, C.force[b].z, C.torque[b].x, C.torque[b].y, C.torque[b].z};
+    std::memcpy(force + 3 * b, a, 24);
+    std::memcpy(torque + 3 * b, a + 3, 24);
+  }
+}
+void* ora_cdf_fluid(void* c) { return &((CoupledDfsph*)c)->D; }
 void ora_world_set_threads(void* w, int32_t threads) { ((World*)w)->threads = threads < 1 ? 1 : threads; }
 void ora_world_step(void* w, double dt, uint64_t nsteps) {
   for (uint64_t s = 0; s < nsteps; ++s) ((World*)w)->step(dt);

@@ -90,6 +90,7 @@ struct DfArgs {
   float dt, inv_dt, inv_dt2, omega, cap;
   int div;   // 1: divergence solver, 0: density solver
   int warm;  // accumulate the warm-start sums (State::DW)
+  float4* FBT;  // per slot: the reaction -m dv_b / dt the update hands to the boundary (w = 1 if any); null: none
 };
 struct DfTiles {
   uint32_t lds_cap[3];  // alpha, stiffness, update
@@ -110,6 +111,16 @@ struct DfSlip {
   int32_t on[4];
   double at[4];
 };
+// Boundary reactions of one pressure update (dfsph.rs:490-522): each fluid
+// particle's FBT spread over the boundary candidates of its 27 reference cells
+// (edge 2h) that see it, weights min(1 / max(r, 1e-9), 1e6), added to RF (per
+// original index, three int64 in units of res) by integer atomics, so sums do
+// not depend on the order.
+void launch_df_react(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* cell_start,
+                     const DfCtl* ctl, const float4* FBT, long long* RF, double inv_ref, double inv_res);
+// BF of every slot from RF (boundary) or 0 (fluid), IDF = ID, ACC = 0.
+void launch_df_forces_out(hipStream_t st, uint32_t N, const State& cur, const long long* RF, double res);
+
 // Elementwise DFSPH passes (pef_dfsph.hip).
 void launch_df_begin(hipStream_t st, DfCtl* c);
 void launch_df_predict(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, float dt);

@@ -1174,6 +1174,82 @@ __global__ void k_df_sparse_alpha(uint32_t N, const SceneDev* __restrict__ sc, S
   AL[s] = n >= 2u ? -a : 0.0f;
 }
 
+// The reaction of fluid slot s over its boundary candidates (see
+// launch_df_react): two scans of the list grid's 5 x 5 x 5 cells around it,
+// the weight sum, then the shares.
+template <bool ADD>
+__device__ inline double df_react_scan(const SceneDev& S, const State& cur, const uint32_t* __restrict__ cell_start,
+                                       uint32_t s, double inv_ref, double wsum, double3 fb, double inv_res,
+                                       long long* __restrict__ RF) {
+#pragma clang fp contract(off)
+  const double xi = cur.X[s], yi = cur.Y[s], zi = cur.Z[s];
+  const int32_t kx = floor_i32(xi * inv_ref), ky = floor_i32(yi * inv_ref);
+  const int32_t kz = S.dim == 2 ? 0 : floor_i32(zi * inv_ref);
+  const int64_t lx = (int64_t)floor_i32(xi * S.inv_cell) - S.ix0, ly = (int64_t)floor_i32(yi * S.inv_cell) - S.iy0;
+  const int64_t lz = S.dim == 2 ? 0 : (int64_t)floor_i32(zi * S.inv_cell) - S.iz0;
+  const int64_t dz0 = S.dim == 2 ? 0 : -2, dz1 = S.dim == 2 ? 0 : 2;
+  double sum = 0.0;
+  for (int64_t dx = -2; dx <= 2; ++dx)
+    for (int64_t dy = -2; dy <= 2; ++dy)
+      for (int64_t dz = dz0; dz <= dz1; ++dz) {
+        const int64_t cx = lx + dx, cy = ly + dy, cz = lz + dz;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= S.nx || cy >= S.ny || cz >= S.nz) continue;
+        const uint32_t c = cell_id(S, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        for (uint32_t j = cell_start[c]; j < cell_start[c + 1]; ++j) {
+          const uint32_t meta = cur.META[j];
+          if (!(meta & kMetaBoundary)) continue;
+          const double xj = cur.X[j], yj = cur.Y[j], zj = cur.Z[j];
+          const int32_t jx = floor_i32(xj * inv_ref), jy = floor_i32(yj * inv_ref);
+          const int32_t jz = S.dim == 2 ? 0 : floor_i32(zj * inv_ref);
+          if (abs(jx - kx) > 1 || abs(jy - ky) > 1 || abs(jz - kz) > 1) continue;
+          if ((meta & kMetaOriented) &&
+              !sees_f64(xj, yj, zj, cur.NX[j], cur.NY[j], cur.NZ[j], xi, yi, zi, S.h_d))
+            continue;
+          const double rx = xj - xi, ry = yj - yi, rz = zj - zi;
+          const double r = sqrt(rx * rx + ry * ry + rz * rz);
+          const double wt = fmin(1.0 / fmax(r, 1e-9), 1e6);
+          if (ADD) {
+            const double q = wt / wsum;
+            unsigned long long* out = reinterpret_cast<unsigned long long*>(RF) + 3ull * cur.ID[j];
+            atomicAdd(out + 0, (unsigned long long)llrint(fb.x * q * inv_res));
+            atomicAdd(out + 1, (unsigned long long)llrint(fb.y * q * inv_res));
+            atomicAdd(out + 2, (unsigned long long)llrint(fb.z * q * inv_res));
+          } else {
+            sum += wt;
+          }
+        }
+      }
+  return sum;
+}
+
+__global__ void k_df_react(uint32_t N, const SceneDev* __restrict__ sc, State cur,
+                           const uint32_t* __restrict__ cell_start, const DfCtl* __restrict__ ctl,
+                           const float4* __restrict__ FBT, long long* __restrict__ RF, double inv_ref, double inv_res) {
+  if (ctl->done) return;  // the update of this iteration did not run
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const float4 f = FBT[s];
+  if (f.w == 0.0f) return;
+  const SceneDev& S = sc[scene_of(cur.META[s])];
+  const double3 fb = make_double3(f.x, f.y, f.z);
+  const double wsum = df_react_scan<false>(S, cur, cell_start, s, inv_ref, 0.0, fb, inv_res, RF);
+  if (wsum > 0.0) df_react_scan<true>(S, cur, cell_start, s, inv_ref, wsum, fb, inv_res, RF);
+}
+
+__global__ void k_df_forces_out(uint32_t N, State cur, const long long* __restrict__ RF, double res) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const uint32_t id = cur.ID[s];
+  cur.IDF[s] = id;
+  cur.ACC[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // DFSPH leaves Particles::a untouched
+  if (cur.META[s] & kMetaBoundary) {
+    const long long* r = RF + 3ull * id;
+    cur.BF[s] = make_float4((float)((double)r[0] * res), (float)((double)r[1] * res), (float)((double)r[2] * res), 0.0f);
+  } else {
+    cur.BF[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
 // One solver iteration's stiffness (dfsph.rs:318-353, 519-621): the density
 // derivative sum m_j (v*_i - v*_j) . grad W on the predicted velocities (B2;
 // a boundary partner's is its own velocity), then
@@ -1267,10 +1343,10 @@ __global__ void __launch_bounds__(kDfThreads) k_df_kappa_tiled(TileArgs ta, cons
 // (K_i + K_j) grad W with K = kappa / rho from the stiffness pass (0 for a
 // boundary partner), capped at max_velocity_correction, added to v* in place
 // (the pass reads no partner velocity).
-template <bool FAST>
+template <bool FAST, bool REACT>
 __device__ inline float3 df_update_walk(const Round& r, uint32_t sub, const float4& ai, float ki, float qu, float aoh,
-                                        const Src<FAST, 2>& src) {
-  float x = 0.0f, y = 0.0f, z = 0.0f;
+                                        const Src<FAST, 2>& src, float3& bnd) {
+  float x = 0.0f, y = 0.0f, z = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
   walk_chunks<kDfLpp, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
     uint32_t e[8];
     chunk_entries(ch, e);
@@ -1288,8 +1364,15 @@ __device__ inline float3 df_update_walk(const Round& r, uint32_t sub, const floa
       x += rx * g;
       y += ry * g;
       z += rz * g;
+      if (REACT) {  // the boundary partners' share of the update
+        const float gb = (aj.w < 0.0f) ? g : 0.0f;
+        bx += rx * gb;
+        by += ry * gb;
+        bz += rz * gb;
+      }
     }
   });
+  bnd = make_float3(bx, by, bz);
   return make_float3(x, y, z);
 }
 
@@ -1318,26 +1401,44 @@ __global__ void __launch_bounds__(kDfThreads) k_df_update_tiled(TileArgs ta, con
       if (base != 0) rd = round_load<kDfLpp, 1>(Tb, ta, base + threadIdx.x / kDfLpp, n_own, sub);
       const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
-      float3 dv = make_float3(0.0f, 0.0f, 0.0f);
+      float3 dv = make_float3(0.0f, 0.0f, 0.0f), db = dv;
       bool fl = false;
       if (rd.act) {
         if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         fl = !is_boundary_w(ai.w);
-        if (fl)
-          dv = fast ? df_update_walk<true>(rd, sub, ai, bi.w, qu, aoh, fsrc)
-                    : df_update_walk<false>(rd, sub, ai, bi.w, qu, aoh, gsrc);
+        if (fl) {
+          if (da.FBT)
+            dv = fast ? df_update_walk<true, true>(rd, sub, ai, bi.w, qu, aoh, fsrc, db)
+                      : df_update_walk<false, true>(rd, sub, ai, bi.w, qu, aoh, gsrc, db);
+          else
+            dv = fast ? df_update_walk<true, false>(rd, sub, ai, bi.w, qu, aoh, fsrc, db)
+                      : df_update_walk<false, false>(rd, sub, ai, bi.w, qu, aoh, gsrc, db);
+        }
       }
       dv.x = group_sum<kDfLpp>(dv.x);
       dv.y = group_sum<kDfLpp>(dv.y);
       dv.z = group_sum<kDfLpp>(dv.z);
+      if (da.FBT) {
+        db.x = group_sum<kDfLpp>(db.x);
+        db.y = group_sum<kDfLpp>(db.y);
+        db.z = group_sum<kDfLpp>(db.z);
+      }
       if (rd.act && sub == 0u && fl) {
         float vx = -da.dt * dv.x, vy = -da.dt * dv.y, vz = -da.dt * dv.z;
         const float s2 = vx * vx + vy * vy + vz * vz;
+        float scale = 1.0f;
         if (da.cap > 0.0f && s2 > da.cap * da.cap) {
           const float sc_ = da.cap / sqrtf(s2);
+          scale = sc_;
           vx *= sc_;
           vy *= sc_;
           vz *= sc_;
+        }
+        if (da.FBT) {  // reaction -m dv_b / dt from the capped boundary share (dfsph.rs:490-499)
+          const float f = -da.dt * scale * (-S.mass * da.inv_dt);
+          const float fx = db.x * f, fy = db.y * f, fz = (S.dim == 2) ? 0.0f : db.z * f;
+          const bool any = db.x * db.x + db.y * db.y + db.z * db.z > 0.0f;
+          da.FBT[o.s] = any ? make_float4(fx, fy, fz, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         const float4 v = cur.B2[o.s];
         float* b2 = reinterpret_cast<float*>(cur.B2) + 4ull * o.s;
@@ -1615,6 +1716,15 @@ void launch_df_alpha(hipStream_t st, const DfTiles& d, const TileArgs& ta, const
 void launch_df_sparse_alpha(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur,
                             const uint32_t* cell_start, float* AL, double inv_ref) {
   PEF_LAUNCH_LDS(k_df_sparse_alpha, blocks_of(N, 256), 256, 0, st, N, sc, cur, cell_start, AL, inv_ref);
+}
+
+void launch_df_react(hipStream_t st, uint32_t N, const SceneDev* sc, const State& cur, const uint32_t* cell_start,
+                     const DfCtl* ctl, const float4* FBT, long long* RF, double inv_ref, double inv_res) {
+  PEF_LAUNCH_LDS(k_df_react, blocks_of(N, 256), 256, 0, st, N, sc, cur, cell_start, ctl, FBT, RF, inv_ref, inv_res);
+}
+
+void launch_df_forces_out(hipStream_t st, uint32_t N, const State& cur, const long long* RF, double res) {
+  PEF_LAUNCH_LDS(k_df_forces_out, blocks_of(N, 256), 256, 0, st, N, cur, RF, res);
 }
 
 void launch_df_kappa(hipStream_t st, const DfTiles& d, const TileArgs& ta, const SceneDev* sc, const State& cur,

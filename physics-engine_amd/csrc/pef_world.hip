@@ -154,6 +154,10 @@ struct DfsphHost {
   double* ERR = nullptr;
   double* part = nullptr;
   float* AL = nullptr;
+  bool react = true;         // boundary reactions (dfsph.rs:490-522)
+  float4* FBT = nullptr;     // per slot: the reaction an update hands to the boundary
+  long long* RF = nullptr;   // per original index: the step's reactions, fixed point in units of res
+  double res = 1.0;          // N per RF unit: (m / dt) 2^-32 of the last step
   double count = 0.0;  // fluid particles
   float floor_ = 0.0f;
   uint32_t last_density = 0, last_divergence = 0;
@@ -865,6 +869,8 @@ int32_t get_field(World& w, int what, double* out, uint64_t len, bool vec) {
   if (!out) return PEF_ERR_NULL;
   uint64_t need = vec ? 3ull * w.N : (uint64_t)w.N;
   if (len < need) return PEF_ERR_BOUNDS;
+  // DFSPH keeps its reactions by original index: lay them out for the current slots
+  if (w.df && (what == 2 || what == 5)) pef::launch_df_forces_out(w.stream, w.N, w.cur, w.df->RF, w.df->res);
   pef::launch_unsort(w.stream, w.N, w.sc_d, w.st_d, w.cur, what, w.stage);
   HIPTRY(hipGetLastError());
   HIPTRY(hipMemcpyAsync(out, w.stage, need * sizeof(double), hipMemcpyDeviceToHost, w.stream));
@@ -1427,12 +1433,18 @@ int32_t df_solve(World& w, double dt, bool div, uint32_t* iters) {
   da.cap = (float)p.max_velocity_correction;
   da.div = div ? 1 : 0;
   da.warm = p.warm_start ? 1 : 0;
+  da.FBT = D.react ? D.FBT : nullptr;
   const pef::TileArgs ta = tile_args(w);
   const int which = div ? 1 : 0;
+  const double inv_ref = 1.0 / (2.0 * p.h), inv_res = 1.0 / D.res;
+  auto update = [&]() {
+    pef::launch_df_update(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+    if (D.react) pef::launch_df_react(w.stream, w.N, w.sc_d, w.cur, w.cell_start, D.ctl, D.FBT, D.RF, inv_ref, inv_res);
+  };
   pef::launch_df_begin(w.stream, D.ctl);
   if (p.warm_start) {  // kappa = warm sum, one pressure update, warm sums cleared
     pef::launch_df_warm_prep(w.stream, w.N, w.cur, which);
-    pef::launch_df_update(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+    update();
     pef::launch_df_warm_zero(w.stream, w.N, w.cur, which);
   }
   const uint32_t max_it = std::max<uint32_t>(div ? p.max_divergence_iters : p.max_density_iters, 1u);
@@ -1444,7 +1456,7 @@ int32_t df_solve(World& w, double dt, bool div, uint32_t* iters) {
     for (uint32_t k = 0; k < chunk; ++k) {
       pef::launch_df_kappa(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
       pef::launch_df_reduce(w.stream, w.N, D.ERR, D.part, D.ctl, D.count, tol, min_it, max_it);
-      pef::launch_df_update(w.stream, D.tiles, ta, w.sc_d, w.cur, da);
+      update();
     }
     HIPTRY(hipGetLastError());
     queued += chunk;
@@ -1468,6 +1480,10 @@ int32_t dfsph_step(World& w, double dt, uint32_t nsteps) {
   pef::DfsphHost& D = *w.df;
   const pef::DfSlip slip = df_slip(D.prm);
   for (uint32_t k = 0; k < nsteps; ++k) {
+    if (D.react) {  // boundary_force is cleared at the start of each step (dfsph.rs:717-719)
+      D.res = D.prm.mass / dt * std::ldexp(1.0, -32);
+      HIPTRY(hipMemsetAsync(D.RF, 0, 3ull * w.N * sizeof(long long), w.stream));
+    }
     pef::launch_df_predict(w.stream, w.N, w.sc_d, w.cur, (float)dt);
     launch_xsph(w);  // v* + XSPH into B2 (dfsph.rs:381-417)
     rc = df_solve(w, dt, false, &D.last_density);
@@ -1480,6 +1496,7 @@ int32_t dfsph_step(World& w, double dt, uint32_t nsteps) {
     if (rc) return rc;
     pef::launch_df_slip(w.stream, w.N, w.sc_d, w.cur, w.ctl_d, slip);
     pef::launch_df_finish(w.stream, w.N, w.cur);
+    pef::launch_df_forces_out(w.stream, w.N, w.cur, D.RF, D.res);
     HIPTRY(hipGetLastError());
     w.t += dt;
   }
@@ -1534,6 +1551,7 @@ int32_t create_dfsph(const PefDfsphParams* dp, const PefParticles* parts, const 
   p.adapt_sound_speed = 0;
   PefOptions o{};
   if (opt) o = *opt;
+  const bool react = !(opt && opt->boundary_force < 0);
   o.boundary_force = -1;  // no boundary lists: every DFSPH pass walks fluid particles only
   // DfsphWorld::new keeps 2-D particles planar (dfsph.rs:148-153)
   std::vector<double> x, v;
@@ -1560,6 +1578,7 @@ int32_t create_dfsph(const PefDfsphParams* dp, const PefParticles* parts, const 
   if (!w.df) return fail(PEF_ERR_PANIC);
   pef::DfsphHost& D = *w.df;
   D.prm = *dp;
+  D.react = react;
   if (D.prm.particle_diameter <= 0.0) D.prm.particle_diameter = dp->h;
   if (pef::df_configure(w.tc, D.tiles) != 0) return fail(PEF_ERR_DEVICE);
   const uint32_t N = w.N;
@@ -1569,6 +1588,10 @@ int32_t create_dfsph(const PefDfsphParams* dp, const PefParticles* parts, const 
   if (e == hipSuccess) e = w.dalloc(&D.ERR, (size_t)N);
   if (e == hipSuccess) e = w.dalloc(&D.part, (size_t)pef::df_reduce_blocks(N));
   if (e == hipSuccess) e = w.dalloc(&D.ctl, 1);
+  if (e == hipSuccess) e = w.dalloc(&D.FBT, (size_t)N + 1);
+  if (e == hipSuccess) e = w.dalloc(&D.RF, 3ull * N);
+  if (e == hipSuccess) e = hipMemsetAsync(D.FBT, 0, ((size_t)N + 1) * sizeof(float4), w.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(D.RF, 0, 3ull * N * sizeof(long long), w.stream);
   if (e == hipSuccess) e = hipMemsetAsync(w.cur.DW, 0, ((size_t)N + 1) * sizeof(float2), w.stream);
   if (e == hipSuccess) e = hipMemsetAsync(D.AL, 0, ((size_t)N + 1) * sizeof(float), w.stream);
   if (e == hipSuccess) e = hipMemsetAsync(D.ERR, 0, (size_t)N * sizeof(double), w.stream);
@@ -1584,7 +1607,10 @@ int32_t create_dfsph(const PefDfsphParams* dp, const PefParticles* parts, const 
   if (e != hipSuccess) return fail(PEF_ERR_PANIC);
   w.density_valid = false;
   rc = df_ensure(w);
-  if (!rc) rc = read_flags(w);
+  if (!rc) {
+    pef::launch_df_forces_out(w.stream, w.N, w.cur, D.RF, D.res);  // zero reactions and accelerations
+    rc = read_flags(w);
+  }
   if (rc) return fail(rc);
   return PEF_OK;
 }

@@ -463,6 +463,13 @@ class CoupledWcsphWorld:
         self.rigid.clear_loads()
 
 
+class CoupledDfsphWorld(CoupledWcsphWorld):
+    """CoupledDfsphWorld (coupling.rs:353-408) over a device DfsphWorld. Its
+    step is the WCSPH one's: pef_world_sync_rigid also brings the lists,
+    densities and alpha of the moved state up to date, which is the
+    reference's rebuild_neighbours, compute_density, compute_alpha."""
+
+
 # --- scenes and measurements (coupling.rs:412-441, 603-981) ----------------------
 def archimedes_box_draft(rho_body: float, rho_fluid: float, height: float) -> float:  # coupling.rs:412-415
     return (rho_body / max(rho_fluid, 1e-12)) * height
@@ -547,6 +554,45 @@ def make_floating_box_wcsph_2d(dx: float, tank_w: float, water_h: float, hx: flo
     params.visc_alpha = 0.15
     fluid = pef.WcsphWorld(parts, params, options)
     w = CoupledWcsphWorld(fluid, [body], coupler, (0.0, -g, 0.0))
+    w.particles, w.params = parts, params
+    return w
+
+
+def make_floating_box_2d(dx: float, tank_w: float, water_h: float, hx: float, hy: float, rho_body: float,
+                         options=None) -> CoupledDfsphWorld:
+    """make_floating_box_2d (coupling.rs:603-686): the DFSPH floating box, the
+    body at its Archimedes draft + dx/2, the water inside it carved out
+    (margin 1.25 dx), tolerances 1 %, 40 + 25 iterations, XSPH 0.15, no warm
+    start, velocity cap 0.8, free-slip planes x >= 0, y >= 0, x <= nx dx."""
+    import pef
+
+    rho0 = 1000.0
+    hsm = 2.2 * dx
+    g = STANDARD_GRAVITY
+    parts = pef.Particles(pef.D2, rho0, pef.mass_calibrated_to_rest(rho0, dx, hsm, pef.D2), hsm)
+    nx = max(int(math.floor(tank_w / dx)), 4)
+    ny = max(int(math.floor(water_h / dx)), 4)
+    pef.fill_box(parts, (0.5 * dx, 0.5 * dx, 0.0), nx, ny, 1, dx)
+    pef.fill_box_boundary(parts, (0.0, 0.0, 0.0), (nx * dx, water_h * 2.5, 0.0), dx, 2)
+    pef.recomputed_akinci_psi(parts)
+    draft = archimedes_box_draft(rho_body, rho0, 2.0 * hy)
+    y_com = (water_h - draft) + hy + 0.5 * dx
+    x_com = 0.5 * tank_w
+    half = (hx, hy, 0.0)
+    carve_fluid_box(parts, (x_com, y_com, 0.0), half, 1.25 * dx)
+    coupler = FluidRigidCoupler(1)
+    coupler.sample_box(parts, 0, half, dx, 1)
+    mass_b = rho_body * (4.0 * hx * hy)
+    body = RigidBody(mass_b).with_position((x_com, y_com, 0.0)).with_inertia(
+        inertia_solid_box(mass_b, 2.0 * hx, 2.0 * hy, 1.0))
+    rigid = RigidWorld([body], (0.0, -g, 0.0))
+    coupler.place(parts, rigid)
+    params = pef.DfsphParams(gravity=(0.0, -g, 0.0), particle_diameter=dx, density_error_tol=0.01,
+                             divergence_error_tol=0.01, max_density_iters=40, max_divergence_iters=25,
+                             xsph_epsilon=0.15, adaptive_dt=False, warm_start=False, max_velocity_correction=0.8,
+                             free_slip_xmin=0.0, free_slip_ymin=0.0, free_slip_xmax=nx * dx)
+    fluid = pef.DfsphWorld(parts, params, options)
+    w = CoupledDfsphWorld(fluid, [body], coupler, (0.0, -g, 0.0))
     w.particles, w.params = parts, params
     return w
 
