@@ -120,3 +120,35 @@ def test_batch_scene_generator_draws_like_the_reference_rng():
         assert (c.nx - 1 + 4) * c.dx <= c.tank[0] + 1e-12
         assert abs((c.ny - 1 + 4) * c.dx - c.tank[1]) < 1e-9
         assert (c.nz - 1 + 4) * c.dx <= c.tank[2] + 1e-12
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every struct of include/pef.h has a ctypes mirror with the same field
+    names, offsets and size, as gcc lays the header out."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "pef.h")).read(), flags=re.S)
+    structs = {}
+    for name, body in re.findall(r"typedef struct (\w+) \{(.*?)\}\s*\1;", txt, flags=re.S):
+        fields = []
+        for decl in filter(None, (d.strip() for d in body.split(";"))):
+            rest = re.match(r"(?:const )?\w+\*?\s+(.*)$", decl).group(1)
+            fields += [re.sub(r"\[.*", "", v).strip("* ") for v in rest.split(",")]
+        structs[name] = fields
+    assert len(structs) >= 7
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pef.h"', "int main(void) {"]
+    for name, fields in structs.items():
+        lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
+        lines += [f'  printf("{name} {f} %zu\\n", offsetof({name}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    c_layout = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        s, f, v = ln.split()
+        c_layout[(s, f)] = int(v)
+    for name, fields in structs.items():
+        cls = getattr(pef, name)
+        assert [f[0] for f in cls._fields_] == fields, name
+        assert C.sizeof(cls) == c_layout[(name, "sizeof")], name
+        for f in fields:
+            assert getattr(cls, f).offset == c_layout[(name, f)], (name, f)
