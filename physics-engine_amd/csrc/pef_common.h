@@ -83,6 +83,11 @@ struct SceneDev {
   float r2l_u;       // (2h (1 + skin) / unit)^2
   float h_u;         // h / unit
   float eh2_u;       // eps h^2 / unit^2
+  // cubic spline in position units (cubic_w_u, cubic_gf_u): with s = unit / h
+  // and a = alpha / h, w_c2 = -1.5 s^2, w_c3 = 0.75 s^3, dw_d0 = 3 s a,
+  // dw_d1 = -2.25 s^2 a, dw_o = 0.75 a
+  float w_c2, w_c3;
+  float dw_d0, dw_d1, dw_o;
 };
 
 // Mutable per-scene state on the device.
@@ -159,16 +164,41 @@ __device__ __host__ inline uint32_t cell_id(const SceneDev& S, uint32_t lx, uint
 __device__ inline uint32_t scene_of(uint32_t meta) { return meta >> kMetaSceneShift; }
 __device__ inline bool is_boundary_w(float w) { return (__float_as_uint(w) >> 31) != 0u; }
 
-// Cubic spline W and dW/dr in FP32 (kernels.rs:79-116); q < 2 is the caller's
-// guarantee (pairs are filtered on r^2 < (2h)^2 first).
-// Both branches are evaluated and selected, so a wave whose lanes straddle
-// q = 1 does not execute two code paths.
-__device__ inline float cubic_w(float q, float alpha) {
-  float t = fmaxf(2.0f - q, 0.0f);
-  float inner = 1.0f - q * q * (1.5f - 0.75f * q);
-  float outer = 0.25f * t * t * t;
-  return alpha * (q < 1.0f ? inner : outer);
+// Cubic spline W and dW/dr in FP32 (kernels.rs:79-116), evaluated on a
+// separation of r position units (q = q_unit r) with the polynomials rewritten
+// in r, so no q is formed:
+//   q < 1 (r < h_u):  W = F (1 + r^2 (w_c2 + w_c3 r)),     -dW/dr / r = dw_d0 + dw_d1 r
+//   1 <= q < 2:       W = 0.25 F (2 - q)^3,                -dW/dr / r = dw_o (2 - q)^2 / r
+// and both exactly 0 for q >= 2 (2 - q is clamped at 0), so list entries beyond
+// 2h and the far sentinel contribute nothing. F is the caller's factor (alpha
+// for W itself). Both branches are evaluated and selected, so a wave whose
+// lanes straddle q = 1 does not execute two code paths.
+struct CubicW {
+  float s, hu, c0, c2, c3, co;
+};
+__device__ inline CubicW cubic_w_consts(const SceneDev& S, float F) {
+  return CubicW{S.q_unit, S.h_u, F, F * S.w_c2, F * S.w_c3, 0.25f * F};
 }
+__device__ inline float cubic_w_u(float r, float r2, const CubicW& C) {
+  const float t = fmaxf(fmaf(-C.s, r, 2.0f), 0.0f);
+  const float inner = fmaf(r2, fmaf(C.c3, r, C.c2), C.c0);
+  const float outer = (t * t) * (t * C.co);
+  return r < C.hu ? inner : outer;
+}
+struct CubicG {
+  float s, hu, d0, d1, o;
+};
+__device__ inline CubicG cubic_g_consts(const SceneDev& S) {
+  return CubicG{S.q_unit, S.h_u, S.dw_d0, S.dw_d1, S.dw_o};
+}
+// -dW/dr / r with 1 / r = rinv: the factor of (x_j - x_i) in grad_i W.
+__device__ inline float cubic_gf_u(float r, float rinv, const CubicG& G) {
+  const float t = fmaxf(fmaf(-G.s, r, 2.0f), 0.0f);
+  const float inner = fmaf(G.d1, r, G.d0);
+  const float outer = (t * t) * (G.o * rinv);
+  return r < G.hu ? inner : outer;
+}
+// dW/dr at q in FP32, in q (kernels.rs:102-116): the DFSPH passes' form.
 __device__ inline float cubic_dwdr(float q, float alpha_over_h) {
   float t = fmaxf(2.0f - q, 0.0f);
   float inner = q * (-3.0f + 2.25f * q);

@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(1024) k_nlist_tiled(TileArgs ta, const Ctl* __
 // the same pass
 // ---------------------------------------------------------------------------
 template <int LPP, bool FAST>
-__device__ inline float density_walk(const Round& r, uint32_t sub, const float4& ai, float r2s, float qu,
+__device__ inline float density_walk(const Round& r, uint32_t sub, const float4& ai, const CubicW& C,
                                      const Src<FAST, 1>& src) {
   float sum = 0.0f, comp = 0.0f;
   walk_chunks<LPP, 2>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
@@ -573,20 +573,21 @@ __device__ inline float density_walk(const Round& r, uint32_t sub, const float4&
     float4 a[8], unused;
 #pragma unroll
     for (int k = 0; k < 8; ++k) src.get(e[k], a[k], unused);
+    float cs = 0.0f;  // the chunk's eight terms, then one compensated add
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const float4 aj = a[k];
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
-      // weight m (fluid) or rho0 psi (boundary); cubic_w is exactly 0 for q >= 2,
-      // so list entries in the skin and the far sentinel add nothing
-      const float term = fabsf(aj.w) * cubic_w(r * qu, 1.0f);
-      const float y = term - comp;  // compensated (Kahan) sum
-      const float tt = sum + y;
-      comp = (tt - sum) - y;
-      sum = tt;
+      // weight m (fluid) or rho0 psi (boundary); W is exactly 0 for q >= 2, so
+      // list entries in the skin and the far sentinel add nothing
+      cs += fabsf(aj.w) * cubic_w_u(r, r2, C);
     }
+    const float y = cs - comp;  // compensated (Kahan) sum over chunks
+    const float tt = sum + y;
+    comp = (tt - sum) - y;
+    sum = tt;
   });
   return sum - comp;
 }
@@ -610,7 +611,7 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
     if (fast) stage_halo<1>(Tb, H, R, CAP, cur.A, nullptr);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
-    const float r2s = S.r2s_u, qu = S.q_unit;
+    const CubicW C = cubic_w_consts(S, 1.0f);  // alpha applies to the whole sum
     // fixed sound speed: the pressure needs no global reduction first (DFSPH has no EOS)
     const bool eos = S.adapt == 0 && S.dfsph == 0;
     const float Bf = (float)(S.rho0_d * st[Tb.scene].c * st[Tb.scene].c / S.gamma_d);
@@ -626,8 +627,8 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
       if (act) {
         if (fast) fsrc.own(o.local, o.s, ai, unused); else gsrc.own(o.local, o.s, ai, unused);
         if (!is_boundary_w(ai.w))
-          part = fast ? density_walk<LPP, true>(rd, sub, ai, r2s, qu, fsrc)
-                      : density_walk<LPP, false>(rd, sub, ai, r2s, qu, gsrc);
+          part = fast ? density_walk<LPP, true>(rd, sub, ai, C, fsrc)
+                      : density_walk<LPP, false>(rd, sub, ai, C, gsrc);
       }
       const float tot = group_sum<LPP>(part);
       if (act && sub == 0u) {
@@ -648,8 +649,8 @@ __global__ void __launch_bounds__(MAXT) k_density_tiled(TileArgs ta, const Scene
 // XSPH velocity smoothing (wcsph.rs:283-323), Jacobi: reads B, writes B2
 // ---------------------------------------------------------------------------
 template <int LPP, bool FAST>
-__device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi, float r2s,
-                                   float qu, float alpha, float two_m, const Src<FAST, 2>& src) {
+__device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& ai, const float4& bi, const CubicW& C,
+                                   const Src<FAST, 2>& src) {
   float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
   walk_chunks<LPP, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
     uint32_t e[8];
@@ -663,10 +664,10 @@ __device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& a
       const float rx = aj.x - ai.x, ry = aj.y - ai.y, rz = aj.z - ai.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float r = __builtin_amdgcn_sqrtf(r2);
-      const float wij = cubic_w(r * qu, alpha);
+      const float wij = cubic_w_u(r, r2, C);  // 2 m W
       // m / rho_bar * W with rho_bar = (rho_i + rho_j) / 2; fluid partners only
       // (W is exactly 0 beyond 2h, so the skin and the sentinel add nothing)
-      float sfac = two_m * wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
+      float sfac = wij * __builtin_amdgcn_rcpf(ai.w + aj.w);
       sfac = (aj.w > 0.0f) ? sfac : 0.0f;
       dvx += (bj.x - bi.x) * sfac;
       dvy += (bj.y - bi.y) * sfac;
@@ -692,7 +693,8 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
-    const float r2s = S.r2s_u, qu = S.q_unit, alpha = S.alpha, two_m = 2.0f * S.mass, eps = S.xsph_eps;
+    const CubicW C = cubic_w_consts(S, 2.0f * S.mass * S.alpha);
+    const float eps = S.xsph_eps;
     const Org org = org_of(Tb);
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
@@ -707,8 +709,8 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
         if (fast) fsrc.own(o.local, o.s, ai, bi); else gsrc.own(o.local, o.s, ai, bi);
         smooth = !is_boundary_w(ai.w) && eps != 0.0f;
         if (smooth)
-          dv = fast ? xsph_walk<LPP, true>(rd, sub, ai, bi, r2s, qu, alpha, two_m, fsrc)
-                    : xsph_walk<LPP, false>(rd, sub, ai, bi, r2s, qu, alpha, two_m, gsrc);
+          dv = fast ? xsph_walk<LPP, true>(rd, sub, ai, bi, C, fsrc)
+                    : xsph_walk<LPP, false>(rd, sub, ai, bi, C, gsrc);
       }
       dv.x = group_sum<LPP>(dv.x);
       dv.y = group_sum<LPP>(dv.y);
@@ -768,7 +770,8 @@ __device__ inline void finish_particle(State& cur, const SceneDev& S, uint32_t s
 }
 
 struct ForceConsts {
-  float r2s, qu, aoh, m, h, vb, eh2, vac;
+  CubicG G;
+  float m, eh2, va, vb;  // va = -2 h alpha c, vb = 2 h^2 beta (h in position units)
 };
 
 template <int LPP, bool FAST, bool IBND>
@@ -796,23 +799,27 @@ __device__ inline float3 force_walk(const Round& r, uint32_t sub, const float4& 
       // there, so no cutoff select is needed
       const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
       const float r = r2 * rinv;
-      const float gf = -cubic_dwdr(r * K.qu, K.aoh) * rinv;
+      const float gf = cubic_gf_u(r, rinv, K.G);
       float sfac;
       if (IBND) {
         sfac = gf * (react * bj.w);  // -m * gw_b * (-(rho0 psi_i) P_j), gw_b = -gw
       } else {
-        // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar,
-        // with one reciprocal of d1 * rho_bar; zero for separating pairs.
+        // Monaghan viscosity (viscosity.rs:51-73): mu = h vr / d1, Pi = mu (-alpha c + beta mu) / rho_bar
+        // with d1 = r^2 + eps h^2 and rho_bar = (rho_i + rho_j) / 2. With u = vr / (d1 (rho_i + rho_j)),
+        // mu / rho_bar = 2 h u and mu = h u (rho_i + rho_j), so Pi = u (va + vb u (rho_i + rho_j)):
+        // one reciprocal per pair; zero for separating pairs.
         const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
         const float vr = -(vx * rx + vy * ry + vz * rz);  // v_ij . (x_i - x_j)
         const float d1 = r2 + K.eh2;
-        const float d2 = 0.5f * (rhoi + aj.w);
-        const float t = K.h * vr * __builtin_amdgcn_rcpf(d1 * d2);  // mu / rho_bar
-        float pi = t * (K.vac + K.vb * (t * d2));
-        pi = (vr < 0.0f) ? pi : 0.0f;
-        const float sf_fluid = gf * (-K.m * ((Pi + bj.w) + pi));
-        const float sf_bnd = gf * (aj.w * Pi);  // gw * (-(rho0 psi_j) P_i)
-        sfac = (aj.w > 0.0f) ? sf_fluid : sf_bnd;
+        const float rs = rhoi + aj.w;
+        const float u = vr * __builtin_amdgcn_rcpf(d1 * rs);
+        float pi = u * (K.va + K.vb * (u * rs));
+        // fluid partner: -m (P_i + P_j + Pi_ij); boundary partner (and the far
+        // sentinel), whose P_j is 0: -(rho0 psi_j) P_i, with Pi_ij dropped
+        const bool fl = aj.w > 0.0f;
+        pi = (fl && vr < 0.0f) ? pi : 0.0f;
+        const float c = fl ? -K.m : aj.w;
+        sfac = gf * (c * ((Pi + bj.w) + pi));
       }
       ax += rx * sfac;
       ay += ry * sfac;
@@ -852,14 +859,11 @@ __global__ void __launch_bounds__(MAXT) k_force_tiled(TileArgs ta, const Ctl* __
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
     ForceConsts K;
-    K.r2s = S.r2s_u;
-    K.qu = S.q_unit;
-    K.aoh = S.alpha * S.inv_h;
+    K.G = cubic_g_consts(S);
     K.m = S.mass;
-    K.h = S.h_u;
-    K.vb = S.visc_beta;
     K.eh2 = S.eh2_u;
-    K.vac = -S.visc_alpha * (float)st[Tb.scene].c;
+    K.va = -2.0f * S.h_u * S.visc_alpha * (float)st[Tb.scene].c;
+    K.vb = 2.0f * S.h_u * S.h_u * S.visc_beta;
     const Org org = org_of(Tb);
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B2, &Tb, ta.N, org};

@@ -215,6 +215,13 @@ void fill_units(pef::SceneDev& S, const PefParams& p, double skin) {
   S.r2l_u = (float)((rl / u) * (rl / u));
   S.h_u = (float)(p.h / u);
   S.eh2_u = (float)(p.visc_eps * p.h * p.h / (u * u));
+  const double pi = 3.14159265358979323846, s = u / p.h;
+  const double a = ((p.dim == 2) ? 10.0 / (7.0 * pi * p.h * p.h) : 1.0 / (pi * p.h * p.h * p.h)) / p.h;
+  S.w_c2 = (float)(-1.5 * s * s);
+  S.w_c3 = (float)(0.75 * s * s * s);
+  S.dw_d0 = (float)(3.0 * s * a);
+  S.dw_d1 = (float)(-2.25 * s * s * a);
+  S.dw_o = (float)(0.75 * a);
 }
 
 // Per-scene device constants from params and the grid placement.

@@ -514,6 +514,34 @@ def estimate_heave_period(ys) -> Optional[float]:
     return s / (len(cross) - 1)
 
 
+def ensemble_heave_period(runs, f_min: float = 0.5, f_max: float = 3.0, pad: int = 8) -> float:
+    """Dominant period of an ensemble of heave records. No counterpart in the
+    reference, whose estimator (estimate_heave_period) reads the up-crossings of
+    one record; at coarse resolution those move with the last bits of the state.
+    Each record is a list of (t, y) samples, all records on the same uniform
+    times. Each y is detrended by its least-squares line, Hann-windowed and
+    zero-padded `pad` times; the power spectra of the records are summed, and
+    the period is 1 / (the frequency of the largest summed power in
+    [f_min, f_max] Hz)."""
+    total, fr, t_ref = None, None, None
+    for ys in runs:
+        a = np.asarray(ys, dtype=np.float64)
+        t, y = a[:, 0] - a[0, 0], a[:, 1]
+        if t_ref is None:
+            t_ref = t
+        elif len(t) != len(t_ref) or not np.allclose(t, t_ref, rtol=0.0, atol=1e-9):
+            raise ValueError("heave records must share their sample times")
+        yd = y - np.polyval(np.polyfit(t, y, 1), t)
+        n = len(yd)
+        p = np.abs(np.fft.rfft(yd * np.hanning(n), pad * n)) ** 2
+        fr = np.fft.rfftfreq(pad * n, t[1] - t[0])
+        total = p if total is None else total + p
+    if total is None:
+        raise ValueError("no heave records")
+    m = (fr >= f_min) & (fr <= f_max)
+    return float(1.0 / fr[m][np.argmax(total[m])])
+
+
 def make_floating_box_wcsph_2d(dx: float, tank_w: float, water_h: float, hx: float, hy: float,
                                rho_body: float, options=None) -> CoupledWcsphWorld:
     """make_floating_box_wcsph_2d (coupling.rs:869-934): a 2-D tank of water

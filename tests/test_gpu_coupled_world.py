@@ -13,8 +13,10 @@ Bars:
     its fluid positions are perturbed by 3e-8 m, FP32's resolution here;
   - Archimedes draft after 2 000 steps of 2.5e-4 s: rel < 0.10 and the
     reference draft within 2 % of the analytic one (coupling_oracles.rs:29-52);
-  - heave period after release: >= 3 up-crossings and rel < 0.15 against
-    2 pi sqrt((m + m_a) / (rho g A)) (coupling_oracles.rs:54-110);
+  - heave period after release: rel < 0.15 against 2 pi sqrt((m + m_a) /
+    (rho g A)) (coupling_oracles.rs:54-110), on the dominant period of eight
+    realisations (the release is chaotic at this resolution; the test says
+    how the f64 oracle's own estimates spread), each with >= 3 up-crossings;
   - closed coupled momentum without walls or gravity: drift < 1e-3 over 50
     steps (coupling_oracles.rs:112-165);
   - no penetration of the floating box: < 2 dx over 800 steps
@@ -110,8 +112,17 @@ def test_floating_box_archimedes_draft():
     assert abs(d_ref - d_an) / d_an < 0.02
 
 
-def test_floating_box_heave_period():
+def _heave_record(seed):
+    """Settle the floating box 2 000 steps, lower it 0.04 m and release it with
+    alpha 0.01 and no XSPH (coupling_oracles.rs:54-110); returns the (t, y)
+    record 0.1 s to 4 s after release. seed > 0 first moves every fluid
+    particle by up to 1e-8 m in x and y (numpy default_rng(seed))."""
     w = CP.make_floating_box_wcsph_2d(DX, TANK_W, WATER_H, HX, HY, RHO_B)
+    if seed:
+        x = w.fluid.positions()
+        fl = w.particles.kind == pef.FLUID
+        x[fl, :2] += np.random.default_rng(seed).uniform(-1e-8, 1e-8, size=(int(fl.sum()), 2))
+        w.fluid.set_state(x=x)
     dt, kick = 2.5e-4, 0.04
     for _ in range(2000):
         w.step(dt)
@@ -127,15 +138,38 @@ def test_floating_box_heave_period():
         w.step(dt)
         samples.append((w.rigid.t, w.rigid.x[0][1]))
     t0 = samples[0][0]
-    win = [(t, y) for t, y in samples if t0 + 0.1 <= t <= t0 + 4.0]
-    mass, a_wp = w.rigid.mass[0], 2.0 * HX
+    return [(t, y) for t, y in samples if t0 + 0.1 <= t <= t0 + 4.0], w.rigid.mass[0]
+
+
+def test_floating_box_heave_period():
+    """The reference's bar (rel < 0.15 against 2 pi sqrt((m + m_a) / (rho g A)))
+    on the dominant period of eight realisations of its scene: one as built and
+    seven with the fluid moved by up to 1e-8 m.
+
+    With 144 fluid particles the release is chaotic, and the reference's
+    up-crossing estimate of one record (coupling.rs:715-740) is a draw from a
+    wide distribution: the f64 oracle gives 0.43 to 1.11 s over fourteen such
+    realisations of this scene, nine of them outside the bar (the reference's
+    notes record one draw, 0.776 s). The summed spectrum of the oracle's
+    fourteen records peaks at 0.800 s (rel 0.045), and that of any eight of
+    them (300 random subsets) within rel 0.091. So the bar applies to the
+    ensemble's spectral peak (pef.coupling.ensemble_heave_period); each
+    record must still oscillate (>= 3 up-crossings of its mean), and its
+    up-crossing estimate is printed."""
+    runs = []
+    for seed in range(8):
+        win, mass = _heave_record(seed)
+        mean = sum(y for _, y in win) / len(win)
+        ncross = sum(1 for i in range(1, len(win)) if win[i - 1][1] - mean <= 0.0 < win[i][1] - mean)
+        print(f"realisation {seed}: up-crossing estimate {CP.estimate_heave_period(win):.4f} s, {ncross} crossings",
+              flush=True)
+        assert ncross >= 3, seed
+        runs.append(win)
+    a_wp = 2.0 * HX
     t_ref = CP.heave_period_with_added_mass(mass, CP.barge_heave_added_mass_2d(1000.0, a_wp), 1000.0, a_wp, G)
-    t_sim = CP.estimate_heave_period(win)
-    mean = sum(y for _, y in win) / len(win)
-    ncross = sum(1 for i in range(1, len(win)) if win[i - 1][1] - mean <= 0.0 < win[i][1] - mean)
+    t_sim = CP.ensemble_heave_period(runs)
     rel = abs(t_sim - t_ref) / t_ref
-    print(f"heave T_sim={t_sim:.4f} T_ref={t_ref:.4f} ncross={ncross} rel={rel:.4f}")
-    assert ncross >= 3
+    print(f"heave ensemble T_sim={t_sim:.4f} T_ref={t_ref:.4f} rel={rel:.4f}")
     assert rel < 0.15
 
 

@@ -172,6 +172,17 @@ def test_heave_helpers_closed_forms():
     ys = list(zip(t, 0.3 + 0.01 * np.sin(2 * np.pi * t / T + 0.3)))
     assert CP.estimate_heave_period(ys) == pytest.approx(T, rel=1e-6)
     assert CP.estimate_heave_period(ys[:7]) is None
+    # the ensemble's spectral peak: records of one period with random phases,
+    # drifts and noise, on the times the heave test samples (0.1 s to 4 s)
+    rng = np.random.default_rng(3)
+    tt = np.arange(0.1, 4.0, 2.5e-4)
+    runs = [list(zip(tt, 0.4 + 0.02 * rng.normal() * tt + 0.01 * np.sin(2 * np.pi * tt / T + rng.uniform(0, 6.3))
+                     + 0.004 * rng.normal(size=len(tt)))) for _ in range(8)]
+    # bin spacing of the 8x padded spectrum near 1 / T: T^2 / (8 * 3.9 s)
+    assert CP.ensemble_heave_period(runs) == pytest.approx(T, abs=T * T / (8 * 3.9))
+    assert CP.ensemble_heave_period(runs[:1]) == pytest.approx(T, abs=T * T / (8 * 3.9))
+    with pytest.raises(ValueError):
+        CP.ensemble_heave_period([runs[0], runs[1][:-5]])
     assert CP.archimedes_box_draft(500.0, 1000.0, 0.16) == pytest.approx(0.08)
     m, a_wp = 19.2, 0.24
     m_a = CP.barge_heave_added_mass_2d(1000.0, a_wp)
