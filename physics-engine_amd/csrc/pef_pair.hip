@@ -270,10 +270,12 @@ __device__ inline const uint4* list_chunks(const uint16_t* LIST, uint32_t s, uin
   return reinterpret_cast<const uint4*>(LIST + (uint64_t)s * cap);
 }
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-// Lists are streamed once per pass: non-temporal loads keep them from evicting
-// the halo data that neighbouring bricks stage through L2.
+// Lists are read once per pass, with ordinary loads: a lane group reads one
+// 64-byte block per step, half of a 128-byte line whose other half it reads
+// next step, and a non-temporal load lets the line leave L2 in between (the
+// pair passes then fetch 25-32 % more bytes from memory).
 __device__ inline uint4 load_chunk(const uint4* lc, uint32_t g) {
-  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(lc + g));
+  const u32x4_t v = *reinterpret_cast<const u32x4_t*>(lc + g);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ inline void chunk_entries(const uint4& c, uint32_t e[8]) {
