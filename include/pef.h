@@ -141,7 +141,9 @@ int32_t pef_default_params(double rest_density, double sound_speed, const double
                            PefParams* out);
 
 /* WcsphWorld::new (wcsph.rs:82-99). Uploads the particles and allocates the
- * device state; `options` may be NULL. */
+ * device state; `options` may be NULL. A world holds at least one particle
+ * (PEF_ERR_ARG for none); a world of boundary particles alone steps without
+ * moving them. */
 int32_t pef_world_create(const PefParams* params, const PefParticles* particles,
                          const PefOptions* options, PefWorld** out);
 

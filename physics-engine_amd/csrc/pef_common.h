@@ -87,7 +87,8 @@ struct SceneDev {
   // and a = alpha / h, w_c2 = -1.5 s^2, w_c3 = 0.75 s^3, dw_d0 = 3 s a,
   // dw_d1 = -2.25 s^2 a, dw_o = 0.75 a
   float w_c2, w_c3;
-  float dw_d0, dw_d1, dw_o;
+  float dw_d0, dw_d1, dw_o;  double mass_d;     // particle mass and |gravity| for the FP64 diagnostics
+  double gmag_d;
 };
 
 // Mutable per-scene state on the device.

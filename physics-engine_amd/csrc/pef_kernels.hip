@@ -332,8 +332,7 @@ __global__ void __launch_bounds__(kDiagThreads) k_diag_partial(uint32_t first, u
                                                                uint32_t scene, State cur, DiagAcc* __restrict__ part) {
   const SceneDev& S = sc[scene];
   DiagAcc a = {};
-  const double m = (double)S.mass;
-  const double gmag = sqrt((double)S.g[0] * S.g[0] + (double)S.g[1] * S.g[1] + (double)S.g[2] * S.g[2]);
+  const double m = S.mass_d, gmag = S.gmag_d;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
     uint32_t s = first + k;
     if (cur.META[s] & kMetaBoundary) continue;

@@ -233,6 +233,8 @@ void fill_scene_physics(pef::SceneDev& S, const PefParams& p, double skin) {
   S.h = (float)p.h;
   S.inv_h = (float)(1.0 / p.h);
   S.mass = (float)p.mass;
+  S.mass_d = p.mass;
+  S.gmag_d = std::sqrt(p.gravity[0] * p.gravity[0] + p.gravity[1] * p.gravity[1] + p.gravity[2] * p.gravity[2]);
   S.rho0 = (float)p.rest_density;
   S.inv_rho0 = (float)(1.0 / p.rest_density);
   const double pi = 3.14159265358979323846;

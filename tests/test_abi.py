@@ -88,6 +88,9 @@ def test_create_rejects_bad_input_before_device_work():
     bad.kind[2] = 7
     cpb = pef._c_particles(bad, keep)
     assert L.pef_world_create(C.byref(pp), C.byref(cpb), None, C.byref(h)) == pef.ERR_ARG
+    empty = pef.Particles(3, 1000.0, 0.001, 0.01)
+    cpe = pef._c_particles(empty, keep)
+    assert L.pef_world_create(C.byref(pp), C.byref(cpe), None, C.byref(h)) == pef.ERR_ARG
     assert L.pef_world_step(None, 1e-4, 1) == pef.ERR_NULL
     n = C.c_uint64()
     assert L.pef_fill_box(3, None, 1, 1, 1, 0.1, None, 0, C.byref(n)) == pef.ERR_NULL

@@ -292,6 +292,47 @@ def test_errors_and_bounds():
     assert e.value.code == pef.ERR_GRID
 
 
+def test_isolated_particle_and_boundary_only_worlds():
+    """Edge cases of WcsphWorld::new/step (wcsph.rs:83-99, 326-347): a fluid
+    particle with no partner has density m W(0) = m / (pi h^3), no pressure,
+    viscous or XSPH term, and falls under gravity by symplectic Euler; a world
+    of boundary particles alone steps without moving and reports no fluid
+    (diagnostics.rs:83-87: mean density error 0), with the oracle's hash."""
+    h, m, dt, k = 0.02, 0.008, 1e-3, 10
+    g = (0.0, 0.0, -9.81)
+    x0, v0 = np.array([0.3, 0.2, 0.5]), np.array([0.1, 0.0, 0.2])
+    one = pef.Particles(3, 1000.0, m, h)
+    one._append([x0], [v0], [pef.FLUID], [0.0], [(0.0, 0.0, 0.0)])
+    prm = pef.WcsphParams.water(1000.0, 20.0, g)
+    W = pef.WcsphWorld(one, prm)
+    for _ in range(k):
+        W.step(dt)
+    rho = W.densities()[0]
+    assert rho == pytest.approx(m / (np.pi * h ** 3), rel=1e-6)
+    gv = np.array(g)
+    want_v = v0 + k * dt * gv
+    want_x = x0 + k * dt * v0 + dt * dt * gv * k * (k + 1) / 2
+    assert np.allclose(W.velocities()[0], want_v, rtol=1e-6, atol=1e-7)
+    assert np.allclose(W.positions()[0], want_x, rtol=0.0, atol=1e-8)
+    assert np.allclose(W.accelerations()[0], gv, rtol=1e-7)
+    d = W.diagnostics()
+    assert d["fluid_count"] == 1 and d["total_mass"] == pytest.approx(m, rel=1e-15)
+
+    shell = pef.Particles(3, 1000.0, m, h)
+    pef.fill_box_boundary(shell, (0.0, 0.0, 0.0), (0.2, 0.1, 0.1), 0.01, 1)
+    pef.recomputed_akinci_psi(shell)
+    B = pef.WcsphWorld(shell, prm)
+    x_before = B.positions()
+    B.step(dt, 3)
+    assert np.array_equal(B.positions(), x_before)
+    assert not B.boundary_force().any()
+    d = B.diagnostics()
+    ref = O.World(dict(dim=3, rest_density=1000.0, mass=m, h=h, sound_speed=20.0, gravity=g), x_before,
+                  shell.kind, psi=shell.psi).diagnostics(9.81)
+    assert d["fluid_count"] == 0 and d["mean_density_error"] == 0.0 and d["total_mass"] == 0.0
+    assert d["hash"] == ref["hash"]
+
+
 def test_list_capacity_overflow_is_reported():
     # A list capacity below the neighbour count must surface as the sticky
     # PEF_ERR_NEIGHBOURS, never as a silently truncated sum.
