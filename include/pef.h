@@ -300,7 +300,8 @@ int32_t pef_dfsph_get_alpha(PefWorld* world, double* out, uint64_t len);
 /* Register the FluidRigidCoupler samples (coupling.rs:267-286): sample k is
  * boundary particle particle[k] (original index) of body body[k] < n_bodies,
  * at body-frame offset local[3k..3k+2] with body-frame normal
- * local_n[3k..3k+2]. Replaces any previous registration. */
+ * local_n[3k..3k+2]. A particle is the sample of one body at most
+ * (PEF_ERR_ARG otherwise). Replaces any previous registration. */
 int32_t pef_world_set_rigid_samples(PefWorld* world, uint64_t n, const uint64_t* particle, const uint32_t* body,
                                     const double* local, const double* local_n, uint32_t n_bodies);
 /* FluidRigidCoupler::sync_from_rigid (coupling.rs:289-305) on the device:

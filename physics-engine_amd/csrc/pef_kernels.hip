@@ -148,6 +148,38 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_sums(const Ctl* __restric
   if (threadIdx.x == 0) block_sums[nb] = total;
 }
 
+// The same exclusive scan (out[n] = the total) by one workgroup, tile after
+// tile with a running carry: for arrays of a few tiles, one launch in place of
+// three, which matters on the many steps where the gated scan has no work.
+constexpr uint32_t kScanSingleTiles = 16;
+__global__ void __launch_bounds__(kScanThreads) k_scan_single(const Ctl* __restrict__ ctl, int always,
+                                                              const uint32_t* __restrict__ in,
+                                                              uint32_t* __restrict__ out, uint32_t n) {
+  if (!always && !ctl->rebuilding) return;
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < n; t0 += kScanTile) {
+    const uint32_t base = t0 + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const uint32_t i = base + k;
+      v[k] = (i < n) ? in[i] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t pre = carry + block_excl_scan(sum, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const uint32_t i = base + k;
+      if (i < n) out[i] = pre;
+      pre += v[k];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
 __global__ void k_scan_add(const Ctl* __restrict__ ctl, int always, uint32_t* __restrict__ out,
                            const uint32_t* __restrict__ block_sums, uint32_t C, uint32_t nb) {
   if (!always && !ctl->rebuilding) return;
@@ -770,10 +802,7 @@ void launch_grid_build(hipStream_t st, const GridArgs& g, int always) {
   const uint32_t bs = 256;
   PEF_LAUNCH(k_cell_count, blocks_for(N, bs), bs, st, N, g.ctl, g.ctl, g.sc, g.meta, g.X, g.Y, g.Z, g.cell_of,
              g.rank_of, g.counts, always);
-  uint32_t nb = blocks_for(C, kScanTile);
-  PEF_LAUNCH(k_scan_local, nb, kScanThreads, st, g.ctl, always, g.counts, g.cell_start, g.block_sums, C);
-  PEF_LAUNCH(k_scan_sums, 1, kScanThreads, st, g.ctl, always, g.block_sums, nb);
-  PEF_LAUNCH(k_scan_add, blocks_for(C + 1, bs), bs, st, g.ctl, always, g.cell_start, g.block_sums, C, nb);
+  launch_scan(st, g.ctl, always, g.counts, g.cell_start, g.block_sums, C);
   PEF_LAUNCH(k_scatter, blocks_for(N, bs), bs, st, N, g.ctl, always, g.cell_of, g.rank_of, g.cell_start, g.perm);
 }
 
@@ -789,6 +818,10 @@ void launch_finalize(hipStream_t st, uint32_t N, const Ctl* ctl, const SceneDev*
 void launch_scan(hipStream_t st, const Ctl* ctl, int always, const uint32_t* in, uint32_t* out,
                  uint32_t* block_sums, uint32_t n) {
   const uint32_t nb = blocks_for(n, kScanTile);
+  if (nb <= kScanSingleTiles) {
+    PEF_LAUNCH(k_scan_single, 1, kScanThreads, st, ctl, always, in, out, n);
+    return;
+  }
   PEF_LAUNCH(k_scan_local, nb, kScanThreads, st, ctl, always, in, out, block_sums, n);
   PEF_LAUNCH(k_scan_sums, 1, kScanThreads, st, ctl, always, block_sums, nb);
   PEF_LAUNCH(k_scan_add, blocks_for(n + 1, 256), 256, st, ctl, always, out, block_sums, n, nb);

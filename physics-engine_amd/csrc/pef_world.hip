@@ -193,6 +193,18 @@ int32_t hip_status(hipError_t e) { return e == hipSuccess ? PEF_OK : PEF_ERR_PAN
 
 bool finite3(const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
+// A temporary device buffer, freed when it leaves scope (also on the early
+// returns of HIPTRY).
+struct DevTmp {
+  void* p = nullptr;
+  DevTmp() = default;
+  DevTmp(const DevTmp&) = delete;
+  DevTmp& operator=(const DevTmp&) = delete;
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 int32_t check_params(const PefParams& p) {
   if (p.dim != 2 && p.dim != 3) return PEF_ERR_ARG;
   if (p.kernel != PEF_KERNEL_CUBIC_SPLINE) return PEF_ERR_UNSUPPORTED;
@@ -892,7 +904,7 @@ int32_t build_probe(World& w) {
   if (w.probe_valid) return PEF_OK;
   w.release_probe();
   // key bounds per scene
-  int32_t* d_bounds = nullptr;
+  DevTmp t_bounds;
   std::vector<int32_t> hb(6ull * w.S);
   for (uint32_t k = 0; k < w.S; ++k) {
     hb[6 * k + 0] = hb[6 * k + 1] = hb[6 * k + 2] = INT32_MAX;
@@ -906,12 +918,12 @@ int32_t build_probe(World& w) {
   }
   HIPTRY(hipMalloc(&w.probe_sc, w.S * sizeof(pef::SceneDev)));
   HIPTRY(hipMemcpyAsync(w.probe_sc, psc.data(), w.S * sizeof(pef::SceneDev), hipMemcpyHostToDevice, w.stream));
-  HIPTRY(hipMalloc(&d_bounds, hb.size() * 4));
+  HIPTRY(hipMalloc(&t_bounds.p, hb.size() * 4));
+  int32_t* d_bounds = static_cast<int32_t*>(t_bounds.p);
   HIPTRY(hipMemcpyAsync(d_bounds, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, w.stream));
   pef::launch_key_bounds(w.stream, w.N, w.probe_sc, w.cur, d_bounds);
   HIPTRY(hipMemcpyAsync(hb.data(), d_bounds, hb.size() * 4, hipMemcpyDeviceToHost, w.stream));
   HIPTRY(hipStreamSynchronize(w.stream));
-  (void)hipFree(d_bounds);
   uint64_t cells = 0;
   for (uint32_t k = 0; k < w.S; ++k) {
     pef::SceneDev& S = psc[k];
@@ -969,18 +981,17 @@ int32_t get_grid(World& w, int32_t* keys, uint64_t* order, int32_t* occupied, ui
   *n_cells = m;
   if (!occupied && !ranges) return PEF_OK;
   if (cap_cells < m || !occupied || !ranges) return PEF_ERR_BOUNDS;
-  int32_t* d_occ = nullptr;
-  uint64_t* d_rng = nullptr;
-  HIPTRY(hipMalloc(&d_occ, 3ull * (m ? m : 1) * 4));
-  HIPTRY(hipMalloc(&d_rng, 2ull * (m ? m : 1) * 8));
+  DevTmp t_occ, t_rng;
+  HIPTRY(hipMalloc(&t_occ.p, 3ull * (m ? m : 1) * 4));
+  HIPTRY(hipMalloc(&t_rng.p, 2ull * (m ? m : 1) * 8));
+  int32_t* d_occ = static_cast<int32_t*>(t_occ.p);
+  uint64_t* d_rng = static_cast<uint64_t*>(t_rng.p);
   pef::launch_compact_cells(w.stream, w.probe_C, w.probe_sc, w.S, w.probe_start, w.probe_flag, w.probe_flagpos,
                             w.probe_sums, d_occ, d_rng, w.ctl_d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(occupied, d_occ, 3ull * m * 4, hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipMemcpyAsync(ranges, d_rng, 2ull * m * 8, hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
-  (void)hipFree(d_occ);
-  (void)hipFree(d_rng);
   return hip_status(e);
 }
 
@@ -1001,11 +1012,12 @@ int32_t get_neighbours(World& w, uint64_t i, uint64_t* buf, uint64_t cap, uint64
   if (rc) return rc;
   rc = build_probe(w);
   if (rc) return rc;
-  uint64_t* d_out = nullptr;
-  uint64_t* d_len = nullptr;
+  DevTmp t_out, t_len;
   uint64_t dcap = cap ? cap : 1;
-  HIPTRY(hipMalloc(&d_out, dcap * 8));
-  HIPTRY(hipMalloc(&d_len, 8));
+  HIPTRY(hipMalloc(&t_out.p, dcap * 8));
+  HIPTRY(hipMalloc(&t_len.p, 8));
+  uint64_t* d_out = static_cast<uint64_t*>(t_out.p);
+  uint64_t* d_len = static_cast<uint64_t*>(t_len.p);
   pef::launch_probe_candidates(w.stream, w.probe_sc, scene, w.probe_keys + 3 * i, w.probe_start, w.probe_order,
                                d_out, cap, d_len);
   uint64_t n = 0;
@@ -1014,8 +1026,6 @@ int32_t get_neighbours(World& w, uint64_t i, uint64_t* buf, uint64_t cap, uint64
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
   if (e == hipSuccess && buf && n <= cap) e = hipMemcpyAsync(buf, d_out, n * 8, hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
-  (void)hipFree(d_out);
-  (void)hipFree(d_len);
   if (e != hipSuccess) return PEF_ERR_PANIC;
   *out_len = n;
   if (!buf || n > cap) return PEF_ERR_BOUNDS;
@@ -1027,12 +1037,11 @@ int32_t get_neighbour_list(World& w, uint64_t i, uint64_t* buf, uint64_t cap, ui
   if (i >= w.N) return PEF_ERR_BOUNDS;
   int32_t rc = ensure_density(w);  // lists exist for the current positions
   if (rc) return rc;
-  uint32_t* d_slot = nullptr;
-  uint64_t* d_out = nullptr;
-  uint32_t* d_len = nullptr;
-  HIPTRY(hipMalloc(&d_slot, (size_t)w.N * 4));
-  HIPTRY(hipMalloc(&d_out, (size_t)w.cap * 8));
-  HIPTRY(hipMalloc(&d_len, 4));
+  void* mem = nullptr;  // slot map, list and length in one allocation
+  HIPTRY(hipMalloc(&mem, (size_t)w.cap * 8 + (size_t)w.N * 4 + 4));
+  uint64_t* d_out = static_cast<uint64_t*>(mem);
+  uint32_t* d_slot = reinterpret_cast<uint32_t*>(d_out + w.cap);
+  uint32_t* d_len = d_slot + w.N;
   pef::launch_slot_of_id(w.stream, w.N, w.cur, d_slot);
   uint32_t slot = 0, n = 0;
   hipError_t e = hipGetLastError();
@@ -1047,9 +1056,7 @@ int32_t get_neighbour_list(World& w, uint64_t i, uint64_t* buf, uint64_t cap, ui
   std::vector<uint64_t> tmp(n);
   if (e == hipSuccess && n) e = hipMemcpyAsync(tmp.data(), d_out, (size_t)n * 8, hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
-  (void)hipFree(d_slot);
-  (void)hipFree(d_out);
-  (void)hipFree(d_len);
+  (void)hipFree(mem);
   if (e != hipSuccess) return PEF_ERR_PANIC;
   std::sort(tmp.begin(), tmp.end());  // presentation order: ascending original index
   *out_len = n;
@@ -1075,17 +1082,16 @@ int32_t diagnostics(World& w, uint32_t scene, PefDiagnostics* out) {
   if (rc) return rc;
   const uint32_t first = (uint32_t)w.scene_first[scene], count = (uint32_t)w.scene_count[scene];
   uint32_t nb = std::max<uint32_t>(1u, std::min<uint32_t>(1024u, (count + 255) / 256));
-  pef::DiagAcc* d_part = nullptr;
-  pef::DiagAcc* d_res = nullptr;
-  HIPTRY(hipMalloc(&d_part, nb * sizeof(pef::DiagAcc)));
-  HIPTRY(hipMalloc(&d_res, sizeof(pef::DiagAcc)));
+  DevTmp t_part, t_res;
+  HIPTRY(hipMalloc(&t_part.p, nb * sizeof(pef::DiagAcc)));
+  HIPTRY(hipMalloc(&t_res.p, sizeof(pef::DiagAcc)));
+  pef::DiagAcc* d_part = static_cast<pef::DiagAcc*>(t_part.p);
+  pef::DiagAcc* d_res = static_cast<pef::DiagAcc*>(t_res.p);
   pef::launch_diag(w.stream, first, count, w.sc_d, scene, w.cur, d_part, nb, d_res);
   pef::DiagAcc a;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(&a, d_res, sizeof(a), hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
-  (void)hipFree(d_part);
-  (void)hipFree(d_res);
   if (e != hipSuccess) return PEF_ERR_PANIC;
   const double m = w.params[scene].mass;
   out->total_mass = m * a.nfluid;
@@ -1262,9 +1268,11 @@ int32_t set_rigid_samples(World& w, uint64_t n, const uint64_t* particle, const 
   if (n && (!particle || !body || !local || !local_n)) return PEF_ERR_NULL;
   if (n > w.N || n_bodies > (1u << 20) || (n && n_bodies == 0)) return PEF_ERR_BOUNDS;
   std::vector<uint32_t> ids(n), order(n), start(n_bodies + 1, 0);
+  std::vector<uint8_t> taken(n ? w.N : 0, 0);
   for (uint64_t k = 0; k < n; ++k) {
     if (particle[k] >= w.N || body[k] >= n_bodies) return PEF_ERR_BOUNDS;
     if (w.h_kind[particle[k]] == PEF_FLUID) return PEF_ERR_ARG;
+    if (taken[particle[k]]++) return PEF_ERR_ARG;  // one sample per boundary particle
     if (!finite3(local + 3 * k) || !finite3(local_n + 3 * k)) return PEF_ERR_ARG;
     ids[k] = (uint32_t)particle[k];
     start[body[k] + 1]++;

@@ -181,6 +181,9 @@ def test_rigid_sample_errors():
         W.sync_rigid(np.zeros((2, 3)), np.tile([1.0, 0, 0, 0], (2, 1)))  # body count differs
     with pytest.raises(pef.PefError):
         W.sync_rigid(np.array([[np.nan, 0, 0]]), np.array([[1.0, 0, 0, 0]]))
+    with pytest.raises(pef.PefError) as e:  # one boundary particle sampled twice
+        W.set_rigid_samples([ids[0], ids[0]], [0, 0], local[:2], local_n[:2], 1)
+    assert e.value.code == pef.ERR_ARG
 
 
 def test_one_sided_visibility_follows_particles_between_list_builds():
