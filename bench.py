@@ -1,8 +1,9 @@
 """Benchmark of the MI355X WCSPH step.
 
 Workload: the 1,048,576-particle dam break (128 x 64 x 128 fluid lattice at
-dx = 0.005 m, h = 2 dx, 163,842 Akinci wall particles, tank 1.6 x 0.32 x 0.8 m,
-dt = 2.5e-5 s), FP32 arithmetic with FP64 position accumulation. One "step" is
+dx = 0.005 m, h = 2 dx, 166,722 Akinci wall particles, tank 1.6 x 0.335 x 0.8 m
+so that a 2 dx gap separates block and walls, dt = 2.5e-5 s), FP32 arithmetic
+with FP64 position accumulation. One "step" is
 one WcsphWorld::step of the whole scene, device-resident.
 
 Prints one JSON line: particle-updates/s over the timed steps, the roofline of
@@ -48,7 +49,7 @@ def kernel_bytes(n_fluid, n_bnd, list_bytes):
 
 
 def step_bytes(n, n_fluid, n_cells, k_mean):
-    """Whole-step algorithmic bytes of the plain SPH step in the survey's model:
+    """Whole-step algorithmic bytes of the plain SPH step (per-pass byte model):
     120 N + (196 + 16 K) N_f + 8 N_cells, K = mean neighbours within 2h of a
     fluid particle (the four pair passes read a 4-byte list entry per pair)."""
     return 120 * n + (196 + 16 * k_mean) * n_fluid + 8 * n_cells
@@ -160,7 +161,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is a design "
-              f"note, not part of this build", file=sys.stderr)
+              f"note, not part of this library", file=sys.stderr)
         sys.exit(2)
     if args.dfsph_steps > 60:
         ap.error("--dfsph-steps is at most 60: the DFSPH run of the dam break leaves the grid near step 70")

@@ -376,7 +376,7 @@ int32_t pef_jitter_positions(uint64_t seed, double amp, uint64_t n, const uint8_
  * for the shells s = floor(2.5 q), q = r / h < 2, and V_j the partner's volume
  * over m / rho0 (rho0 / rho_j for fluid, psi_j rho0 / m for boundary). It is
  * trained on states of the plain step; the plain force pass is its fallback
- * and the judge of its error. */
+ * and the measure of its error. */
 #define PEF_MLP_FEATURES 32
 #define PEF_MLP_OUTPUTS 3
 #define PEF_MLP_PARAMS 37635

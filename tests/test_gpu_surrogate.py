@@ -10,7 +10,7 @@ Bars:
   - the features agree with the oracle's brute-force features on the device's
     own state to 2e-5 + 1e-4 |f| (particles with a partner within 1e-5 of a
     shell edge excluded), and sampled data sets hold exactly those rows;
-  - with the plain force as judge: an untrained network fails every check and
+  - with the plain force as the check: an untrained network fails every check and
     the run is then bit-identical to the plain step; a network
     trained on 24 000 samples of four jittered small dam breaks keeps the
     density at least 3x closer to the plain trajectory than no network at all

@@ -88,7 +88,7 @@ def test_seed_dam_break_particles_layout():
 
 def test_sample_box_matches_the_reference_layout():
     # coupling.rs:103-231: per layer 2 (nx+1)(ny+1) + 2 (nx+1)(nz-1) + 2 (ny-1)(nz-1) samples;
-    # the survey's 0.2 x 0.2 x 0.3 m box at dx 5 mm has 12 802
+    # the 8M configuration's 0.2 x 0.2 x 0.3 m obstacle box at dx 5 mm has 12 802
     from pef import scenes
     P = pef.Particles(pef.D3, 1000.0, 0.125, 0.01)
     n = scenes.sample_box(P, (1.6, 0.3275, 0.15), (0.1, 0.1, 0.15), 0.005)
