@@ -150,8 +150,10 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_sums(const Ctl* __restric
 
 // The same exclusive scan (out[n] = the total) by one workgroup, tile after
 // tile with a running carry: for arrays of a few tiles, one launch in place of
-// three, which matters on the many steps where the gated scan has no work.
-constexpr uint32_t kScanSingleTiles = 16;
+// three on the many steps where the gated scan has no work. It costs about
+// 6.5 us per tile when it runs (the three-kernel scan about 20 us in all), so
+// it is used up to 4 tiles.
+constexpr uint32_t kScanSingleTiles = 4;
 __global__ void __launch_bounds__(kScanThreads) k_scan_single(const Ctl* __restrict__ ctl, int always,
                                                               const uint32_t* __restrict__ in,
                                                               uint32_t* __restrict__ out, uint32_t n) {

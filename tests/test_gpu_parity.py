@@ -46,7 +46,7 @@ def test_grid_and_candidates_bit_exact(cfg):
 
 def test_many_tile_scans_on_a_large_grid():
     """A grid margin of 48 cells gives the list grid and the reference grid
-    well over 16 scan tiles of 8 192 cells, so their prefix sums take the tiled
+    well over 4 scan tiles of 8 192 cells, so their prefix sums take the tiled
     three-kernel scan instead of the one-workgroup scan: the reference grid is
     still the oracle's bit for bit, and densities at creation and after 5 steps
     are those of the default-margin world to FP32 summation order (1e-6 RMS; a
