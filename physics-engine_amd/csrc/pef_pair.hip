@@ -657,7 +657,7 @@ __device__ inline float3 xsph_walk(const Round& r, uint32_t sub, const float4& a
   // instructions per pair; the cancellation costs about 1e-7 |v| s, below the
   // FP32 resolution of the velocities the result is added to
   float svx = 0.0f, svy = 0.0f, svz = 0.0f, ss = 0.0f;
-  walk_chunks<LPP, 1>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
+  walk_chunks<LPP, 2>(r.lc, list_chunks_used(r.cnt), sub, r.c0, r.c1, [&](const uint4 ch) {
     uint32_t e[8];
     chunk_entries(ch, e);
     float4 a[8], b[8];
@@ -695,7 +695,7 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
     const bool fast = H <= CAP;
     const uint32_t n_own = Tb.s1 - Tb.s0;
     const uint32_t ppr = blockDim.x / LPP, sub = threadIdx.x % LPP;
-    Round rd = round_load<LPP, 1>(Tb, ta, threadIdx.x / LPP, n_own, sub);
+    Round rd = round_load<LPP, 2>(Tb, ta, threadIdx.x / LPP, n_own, sub);
     if (fast) stage_halo<2>(Tb, H, R, CAP, cur.A2, cur.B);
     __syncthreads();
     const SceneDev& S = sc[Tb.scene];
@@ -705,7 +705,7 @@ __global__ void __launch_bounds__(MAXT) k_xsph_tiled(TileArgs ta, const SceneDev
     const Src<true, 2> fsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     const Src<false, 2> gsrc{R, CAP, cur.A2, cur.B, &Tb, ta.N, org};
     for (uint32_t base = 0; base < n_own; base += ppr) {
-      if (base != 0) rd = round_load<LPP, 1>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
+      if (base != 0) rd = round_load<LPP, 2>(Tb, ta, base + threadIdx.x / LPP, n_own, sub);
       const bool act = rd.act;
       const Own o = rd.o;
       float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bi = ai;
